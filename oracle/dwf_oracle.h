@@ -1,0 +1,95 @@
+/*
+ * dwf_oracle.h -- CPU oracle for the domain-wall HMC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
+ * The product (lqcd-dwf-hmc_amd/) never links, loads or calls it.
+ *
+ * It restates, in plain scalar C++ (g++, OpenMP, no other dependency), the
+ * algorithm of the reference's corelib even-odd domain-wall operator and of the
+ * solvers and smearing that sit on the hot path.  Every entry point cites the
+ * reference file:line it follows (paths relative to the reference root).
+ *
+ * Layouts are the reference's corelib interchange formats:
+ *   - 4D even-odd spinor:  Field(24, V/2, 1), element (in, site2), in = ri + 2*(c + 3*d)
+ *   - 5D even-odd spinor:  Field(24, V/2, Ls), offset in + 24*(site2 + (V/2)*s)
+ *     (src/lib/Field/field.h:64-66)
+ *   - gauge field:         Field_G(18, V, 4) lexical, offset ab + 18*(site + V*mu),
+ *     ab = ri + 2*(b + 3*a) for the row-major 3x3 matrix U[a][b]
+ *     (src/lib/Tools/mat_SU_N.h, src/lib/Fopr/Imp/fopr_Wilson_impl_SU3-inc.h:30-60)
+ *   - lexical site  = x + Nx*(y + Ny*(z + Nz*t))    (src/lib/Field/index_lex.h:58)
+ *   - even-odd site = x/2 + Nx/2*(y + Ny*(z + Nz*t)), parity (x+y+z+t)%2
+ *     (src/lib/Field/index_eo.h:66, src/lib/Field/index_eo.cpp)
+ *
+ * Parity of each function is pinned by tests/test_oracle_*.py (closed forms and
+ * invariants, since the reference itself cannot be compiled here).
+ */
+#ifndef DWF_ORACLE_H
+#define DWF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_dwf_s* orc_dwf;
+
+/* Operator construction.  prec: 0 = double, 1 = float (the float operator of
+ * the mixed-precision inner solver).  b, c: Ls coefficients each. */
+orc_dwf orc_dwf_create(const int dims[4], int Ls, double mq, double M0,
+                       const double* b, const double* c, double alpha,
+                       const int bc[4], int prec);
+void orc_dwf_destroy(orc_dwf d);
+/* U: lexical Field_G (18, V, 4), double. Copied. */
+void orc_dwf_set_config(orc_dwf d, const double* U);
+int  orc_dwf_nvol2(orc_dwf d);
+int  orc_dwf_ls(orc_dwf d);
+
+/* Operator application on double 5D even-odd fields (for a float operator the
+ * input is rounded to float, the operator runs in float and the output is
+ * widened back to double).  Returns 0, or -1 for an unknown mode. */
+int orc_mult(orc_dwf d, const char* mode, double* v, const double* w);
+int orc_mult_dag(orc_dwf d, const char* mode, double* v, const double* w);
+void orc_mult_gm5(orc_dwf d, double* v, const double* w);
+void orc_mult_gm5R(orc_dwf d, double* v, const double* w);
+/* 4D Wilson even-odd hop of the kernel operator, -kappa * sum_mu (...), on a
+ * Field(24, V/2, 1) slice; ieo = parity of the output. */
+void orc_wilson_meo(orc_dwf d, double* v4, const double* w4, int ieo);
+
+/* Solvers.  mode: operator mode used as A (normally "DdagD").
+ * init_mode: 0 = RHS (x0 = b), 2 = ZERO.  Returns 0 when converged. */
+int orc_cg_solve(orc_dwf d, const char* mode, double* x, const double* b,
+                 int niter, double stop_cond2, int init_mode,
+                 int* nconv, double* diff);
+/* FP64 Richardson (outer, on d64) preconditioned by a float CG (on d32)
+ * solved to stop_inner2 from a zero guess. */
+int orc_richardson_solve(orc_dwf d64, orc_dwf d32, const char* mode,
+                         double* x, const double* b, int niter_outer,
+                         double stop_outer2, int niter_inner, double stop_inner2,
+                         int* nconv, double* diff, int* inner_iters);
+/* Multishift CG on (A + sigma_i), A = mult(mode).  x: nshift pointers. */
+int orc_shiftcg_solve(orc_dwf d, const char* mode, int nshift,
+                      const double* sigma, double** x, const double* b,
+                      int niter, double stop_cond2, int* nconv, double* diff);
+
+/* BLAS helpers on double arrays of n reals. */
+double orc_norm2(const double* x, long n);
+double orc_dot(const double* x, const double* y, long n);
+
+/* Gauge: one APE-staple + stout projection level, U_out = exp(iQ) U_in with
+ * Q from rho * staples.  Lexical Field_G (18, V, 4) in/out. */
+void orc_stout_smear(const int dims[4], double rho, double* Uout,
+                     const double* Uin);
+/* Average plaquette, normalised to 1 for the unit field. */
+double orc_plaquette(const int dims[4], const double* U);
+/* exp(iQ) of one anti-hermitian traceless iQ (18 reals) by Cayley-Hamilton,
+ * and by a 32-term Taylor series (the brute-force check). */
+void orc_exp_iQ(double* e, const double* iQ);
+void orc_exp_iQ_taylor(double* e, const double* iQ);
+
+/* Number of OpenMP threads the oracle uses. */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+"""Writes the committed fixtures under tests/golden/ from the reference's data files.
+
+Run in the build container only (the reference tree is not on the GPU box):
+    python tests/golden/make_fixtures.py
+It copies data, never code:
+  * conf_04040408.npy      the 4x4x4x8 gauge field of conf_04040408.txt, in file
+                           order (sites x 4 directions x 18 reals), float64;
+  * rational/*             the nine parameter_rational.* coefficient files of
+                           benchmark_hmc_dwf/ (plain text, copied verbatim);
+  * known_answers.json     numbers printed by the reference's sample runs.
+"""
+import json
+import os
+import shutil
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def main():
+    vals = np.loadtxt(os.path.join(REF, "conf_04040408.txt"), dtype=np.float64)
+    assert vals.size == 4 * 4 * 4 * 8 * 4 * 18
+    np.save(os.path.join(HERE, "conf_04040408.npy"), vals.reshape(-1, 4, 18))
+    rdir = os.path.join(HERE, "rational")
+    os.makedirs(rdir, exist_ok=True)
+    src = os.path.join(REF, "benchmark_hmc_dwf")
+    for f in sorted(os.listdir(src)):
+        if f.startswith("parameter_rational."):
+            shutil.copyfile(os.path.join(src, f), os.path.join(rdir, f))
+    known = {
+        "plaq_init_8x8x8x8_from_conf_04040408": {
+            "value": 0.62512994697494162,
+            "source": "sample_x86_corelib/run.log:725 (plaq = 6.2512994697494162e-01)",
+        },
+        "plaq_init_8x8x8x8_sample_1gpu": {
+            "value": 0.62512994697494251,
+            "source": "sample_1gpu/run.log:1558 (plaq = 6.2512994697494251e-01)",
+        },
+    }
+    with open(os.path.join(HERE, "known_answers.json"), "w") as f:
+        json.dump(known, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
