@@ -1,0 +1,198 @@
+/*
+ * dwf_hip.h -- C ABI of libdwfhip, the MI355X (gfx950) implementation of the
+ * Bridge++ domain-wall HMC hot path.
+ *
+ * Every entry point returns an int status (DWF_OK = 0) instead of exiting; the
+ * message of the last failure on the calling thread is dwf_last_error().
+ * Plain pointers and sizes only.  "host" pointers are ordinary CPU memory;
+ * "device" pointers are HIP device memory on the current device, e.g. from
+ * dwf_field_alloc().  All device work is ordered on one library stream
+ * (dwf_stream()); calls that return host values synchronise it.
+ *
+ * Interchange formats at the boundary are the reference's corelib layouts:
+ *   gauge field  Field_G(18, V, 4): double U[mu][site][ab], ab = ri+2*(b+3*a),
+ *                lexical site x + Nx*(y + Ny*(z + Nz*t))   (src/lib/Field/field.h:64-66)
+ *   5D eo spinor Field(24, V/2, Ls): double w[s][site2][in], in = ri+2*(c+3*d),
+ *                site2 = x/2 + Nx/2*(y + Ny*(z + Nz*t))    (src/lib/Field/index_eo.h:66)
+ * The device-resident field of an operator ("AFIELD") has its own layout
+ * (site-major, spin-colour, Ls innermost) and precision; only convert/reverse
+ * move data between the two.
+ */
+#ifndef DWF_HIP_H
+#define DWF_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dwf_status {
+  DWF_OK = 0,
+  DWF_ERR_INVALID_ARG = 1,
+  DWF_ERR_UNKNOWN_MODE = 2,   /* reference: vout.crucial + abort() */
+  DWF_ERR_HIP = 3,
+  DWF_ERR_NOT_CONVERGED = 4,  /* reference: logged, nconv = -1 */
+  DWF_ERR_NO_CONFIG = 5,
+  DWF_ERR_UNSUPPORTED = 6,
+  DWF_ERR_NO_DEVICE = 7
+};
+
+enum dwf_precision {
+  DWF_F64 = 0,   /* double storage, double arithmetic */
+  DWF_F32 = 1,   /* float storage, float arithmetic */
+  DWF_F16 = 2,   /* half storage (spinors and links), float arithmetic */
+  DWF_BF16 = 3   /* bfloat16 storage, float arithmetic */
+};
+
+enum dwf_initial_guess {   /* src/lib_alt/Solver/asolver.h InitialGuess */
+  DWF_GUESS_RHS = 0,
+  DWF_GUESS_GIVEN = 1,
+  DWF_GUESS_ZERO = 2
+};
+
+const char* dwf_last_error(void);
+const char* dwf_version(void);
+/* Library stream (hipStream_t as void*), for callers that time or order work. */
+void* dwf_stream(void);
+/* Block until all library work has finished. */
+int dwf_synchronize(void);
+
+/* ------------------------------------------------------------------------
+ * AFopr_Domainwall_5din_eo  (src/lib/Fopr/afopr.h:47-277 interface;
+ * operator of src/lib_alt_QXS/Fopr/afopr_Domainwall_5din_eo-tmpl.h and
+ * src/lib/Fopr/afopr_Domainwall_eo-tmpl.h)
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_fopr_s* dwf_fopr;
+
+typedef struct {
+  int lattice_size[4];        /* Lx Ly Lz Lt (one GPU holds the lattice)        */
+  int extent_of_5th_dimension;/* Ls                                             */
+  double quark_mass;          /* mq                                             */
+  double domain_wall_height;  /* M0                                             */
+  const double* coefficient_b;/* Ls values, or NULL to use b_uniform            */
+  const double* coefficient_c;/* Ls values, or NULL to use c_uniform            */
+  double b_uniform, c_uniform;/* coefficient_b / coefficient_c of the YAML      */
+  double parameter_alpha;     /* 1.0 unless the Accel alpha form is wanted      */
+  int boundary_condition[4];  /* +1 periodic, -1 antiperiodic, 0 Dirichlet      */
+  int precision;              /* enum dwf_precision                             */
+} dwf_fopr_params;
+
+/* AFopr::New("Domainwall_5din_eo", params) (afopr_Domainwall_5din_eo.h:214-221) */
+int dwf_fopr_create(const dwf_fopr_params* params, dwf_fopr* out);
+int dwf_fopr_destroy(dwf_fopr op);
+/* set_config(Field*) (afopr_Domainwall_5din_eo-tmpl.h:367-424): U host Field_G. */
+int dwf_fopr_set_config(dwf_fopr op, const double* U_host);
+/* Same, U already in device memory in the Field_G layout. */
+int dwf_fopr_set_config_device(dwf_fopr op, const double* U_device);
+/* set_mode/get_mode: D, Ddag, DdagD, DDdag, H, Hdag, Deo, Doe, Dee, Doo,
+ * Dee_inv, Doo_inv (afopr_Domainwall_eo-tmpl.h:384-431). */
+int dwf_fopr_set_mode(dwf_fopr op, const char* mode);
+const char* dwf_fopr_get_mode(dwf_fopr op);
+/* mult(v, w) / mult_dag(v, w) in the current mode; v, w device fields. */
+int dwf_fopr_mult(dwf_fopr op, void* v, const void* w);
+int dwf_fopr_mult_dag(dwf_fopr op, void* v, const void* w);
+/* mult(v, w, mode) / mult_dag(v, w, mode) (afopr_Domainwall_eo-tmpl.h:475-525) */
+int dwf_fopr_mult_mode(dwf_fopr op, void* v, const void* w, const char* mode);
+int dwf_fopr_mult_dag_mode(dwf_fopr op, void* v, const void* w, const char* mode);
+/* mult_gm5 (gamma_5 on every s) and mult_gm5R (with s -> Ls-1-s reflection) */
+int dwf_fopr_mult_gm5(dwf_fopr op, void* v, const void* w);
+int dwf_fopr_mult_gm5R(dwf_fopr op, void* v, const void* w);
+/* field_nin() = 24*Ls, field_nvol() = V/2, field_nex() = 1 */
+int dwf_fopr_field_nin(dwf_fopr op);
+int dwf_fopr_field_nvol(dwf_fopr op);
+int dwf_fopr_field_nex(dwf_fopr op);
+/* bytes of one device field of this operator */
+size_t dwf_fopr_field_bytes(dwf_fopr op);
+int dwf_fopr_precision(dwf_fopr op);
+/* flop_count(mode): the reference's flop model
+ * (afopr_Domainwall_5din_eo-tmpl.h:1196-1250). */
+double dwf_fopr_flop_count(dwf_fopr op, const char* mode);
+/* convert(AFIELD& v, const Field& w) / reverse(Field& v, const AFIELD& w)
+ * (afopr_Domainwall_5din_eo-tmpl.h:478-567): corelib Field(24, V/2, Ls) double
+ * host array <-> device field of this operator. */
+int dwf_fopr_convert(dwf_fopr op, void* v_device, const double* w_host);
+int dwf_fopr_reverse(dwf_fopr op, double* v_host, const void* w_device);
+/* Same with the corelib array in device memory. */
+int dwf_fopr_convert_device(dwf_fopr op, void* v_device, const double* w_corelib_device);
+int dwf_fopr_reverse_device(dwf_fopr op, double* v_corelib_device, const void* w_device);
+
+/* ------------------------------------------------------------------------
+ * AField BLAS-1 on device fields of an operator (src/lib_alt_Accel/Field/
+ * afield-tmpl.h:637-742; reductions in double).
+ * ---------------------------------------------------------------------- */
+int dwf_field_alloc(dwf_fopr op, void** field);
+int dwf_field_free(void* field);
+int dwf_field_set_zero(dwf_fopr op, void* x);
+int dwf_field_copy(dwf_fopr op, void* y, const void* x);
+int dwf_field_axpy(dwf_fopr op, void* y, double a, const void* x);   /* y += a x   */
+int dwf_field_aypx(dwf_fopr op, double a, void* y, const void* x);   /* y = a y + x */
+int dwf_field_scal(dwf_fopr op, void* x, double a);
+int dwf_field_norm2(dwf_fopr op, const void* x, double* result);
+int dwf_field_dot(dwf_fopr op, const void* x, const void* y, double* result); /* Re<x,y> */
+int dwf_field_dotc(dwf_fopr op, const void* x, const void* y, double* re, double* im);
+/* convert_h: field of op_from -> field of op_to (precision cast;
+ * aprecond_Mixedprec-tmpl.h:55-108).  Both operators on the same lattice. */
+int dwf_field_convert_precision(dwf_fopr op_to, void* v, dwf_fopr op_from, const void* w);
+
+/* ------------------------------------------------------------------------
+ * Solvers.  ASolver<AFIELD> (src/lib_alt/Solver/asolver.h:22-79).
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_solver_s* dwf_solver;
+
+/* ASolver_CG (asolver_CG-tmpl.h:118-234) on op->mult in op's current mode.
+ * Stops on |r|^2/|b|^2 < stop_cond (convergence_criterion_squared). */
+int dwf_solver_create_cg(dwf_fopr op, int max_iter, double stop_cond,
+                         int initial_guess, dwf_solver* out);
+/* ASolver_Richardson (asolver_Richardson-tmpl.h:131-213) on op_d with
+ * APrecond_Mixedprec (aprecond_Mixedprec-tmpl.h:55-108) running ASolver_CG on
+ * op_f from a zero guess to stop_cond_inner, as Fprop_alt_Standard_eo_Richardson
+ * sets it up (fprop_alt_Standard_eo_Richardson-tmpl.h:27-102, inner 1e-8). */
+int dwf_solver_create_richardson(dwf_fopr op_d, dwf_fopr op_f, int max_iter,
+                                 double stop_cond, int max_iter_inner,
+                                 double stop_cond_inner, dwf_solver* out);
+/* solve(x, b, nconv, diff): diff = |A x - b| / |b| recomputed in the solver's
+ * outer precision.  Returns DWF_ERR_NOT_CONVERGED (nconv = -1) when the
+ * iteration limit is hit; x then holds the last iterate. */
+int dwf_solver_solve(dwf_solver s, void* x, const void* b, int* nconv, double* diff);
+/* Iterations of the inner solver summed over the last solve (0 for CG). */
+int dwf_solver_inner_iterations(dwf_solver s);
+int dwf_solver_destroy(dwf_solver s);
+
+/* AShiftsolver_CG::solve (src/lib/Solver/ashiftsolver_CG-tmpl.h:89-329):
+ * multishift CG on (A + sigma_i), A = op->mult in op's current mode, with the
+ * reference's shift truncation.  x: nshift device fields.  nconv follows the
+ * reference's count; diff = max_i |(A + sigma_i) x_i - b| / |b|. */
+int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond,
+                             int nshift, const double* sigma, void** x,
+                             const void* b, int* nconv, double* diff);
+
+/* ------------------------------------------------------------------------
+ * Stout smearing: Director_alt_Smear + ASmear_APE + AProjection_Stout_SU3
+ * (src/lib_alt/Smear/director_alt_Smear-tmpl.h:19-88,273-286,
+ *  src/lib_alt_QXS/Smear/asmear_APE-tmpl.h:183-223,
+ *  src/lib_alt_QXS/Smear/aprojection_Stout_SU3-tmpl.h:477-553)
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_smear_s* dwf_smear;
+
+int dwf_smear_create(const int lattice_size[4], double rho_uniform,
+                     int number_of_smearing, dwf_smear* out);
+int dwf_smear_destroy(dwf_smear sm);
+/* Director::set_config: keeps U (host Field_G) and computes all levels. */
+int dwf_smear_set_config(dwf_smear sm, const double* U_host);
+int dwf_smear_set_config_device(dwf_smear sm, const double* U_device);
+/* Copy smearing level i (0 = thin links) to a host Field_G. */
+int dwf_smear_get_config(dwf_smear sm, int level, double* U_host);
+/* Device pointer of level i (valid until the next set_config / destroy). */
+const double* dwf_smear_config_device(dwf_smear sm, int level);
+/* AFopr_Smeared_alt::set_config (src/lib/Fopr/afopr_Smeared_alt-tmpl.h:91-98):
+ * smear U, then give the top level to the kernel operator. */
+int dwf_fopr_set_config_smeared(dwf_fopr op, dwf_smear sm, const double* U_host);
+
+/* Average plaquette of a host Field_G (1 for the unit field). */
+int dwf_plaquette(const int lattice_size[4], const double* U_host, double* result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

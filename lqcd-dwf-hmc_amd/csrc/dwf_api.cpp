@@ -1,0 +1,1236 @@
+// dwf_api.cpp -- the extern "C" ABI of libdwfhip (see include/dwf_hip.h).
+//
+// Host side of the operator, solvers and smearing.  Each operator mode is a
+// short chain of the fused k_dwf kernel (hop or site-local core with a
+// per-chirality Ls x Ls epilogue), chosen below so that every Ls-mixing part
+// of the even-odd Moebius operator is folded into a neighbouring hop:
+//
+//   D     = 1 - Di k Hop_eo Mp Di k Hop_oe Mp              (k = -kappa (4 - M0))
+//   Ddag  = 1 - Mp^T k Hop^+_eo Di^T Mp^T k Hop^+_oe Di^T
+// with Mp the 5th-dimension part of D_eo (afopr_Domainwall_eo-tmpl.h:765-828),
+// Di = D_ee^-1 (its L_inv/U_inv, :1022-1117) and Hop^+ = g5 Hop g5.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dwf_hip.h"
+#include "dwf_kernels.h"
+
+using namespace dwf;
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(DWF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// library context: stream and reduction scratch
+// ---------------------------------------------------------------------------
+struct Ctx {
+  bool ok = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  RedScratch red{};
+  double* host_result = nullptr;  // pinned
+  // multishift scratch
+  double* sh_partial = nullptr;
+  unsigned* sh_counters = nullptr;
+  double* sh_pp = nullptr;
+  double* sh_coef = nullptr;
+  void** sh_ptrs = nullptr;       // device array: x[0..n), p[0..n)
+  double* host_coef = nullptr;    // pinned
+  int sh_cap = 0;
+};
+static Ctx g_ctx;
+static std::mutex g_mtx;
+
+static int ctx_init() {
+  if (g_ctx.ok) return DWF_OK;
+  std::lock_guard<std::mutex> lk(g_mtx);
+  if (g_ctx.ok) return DWF_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return fail(DWF_ERR_NO_DEVICE, "no HIP device visible");
+  HIPCHK(hipGetDevice(&g_ctx.device));
+  HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&g_ctx.red.partial, sizeof(double) * kRedBlocks * 4));
+  HIPCHK(hipMalloc(&g_ctx.red.counter, sizeof(unsigned) * 4));
+  HIPCHK(hipMemset(g_ctx.red.counter, 0, sizeof(unsigned) * 4));
+  HIPCHK(hipMalloc(&g_ctx.red.result, sizeof(double) * 16));
+  HIPCHK(hipHostMalloc(&g_ctx.host_result, sizeof(double) * 64, hipHostMallocDefault));
+  g_ctx.ok = true;
+  return DWF_OK;
+}
+
+#define CTX()                           \
+  do {                                  \
+    int rc_ = ctx_init();               \
+    if (rc_ != DWF_OK) return rc_;      \
+  } while (0)
+
+static int fetch(int n, double* out, int slot = 0) {
+  HIPCHK(hipMemcpyAsync(g_ctx.host_result, g_ctx.red.result + slot, sizeof(double) * n,
+                        hipMemcpyDeviceToHost, g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  for (int i = 0; i < n; ++i) out[i] = g_ctx.host_result[i];
+  return DWF_OK;
+}
+
+static int ensure_shift_scratch(int nshift) {
+  if (g_ctx.sh_cap >= nshift) return DWF_OK;
+  if (g_ctx.sh_partial) {
+    (void)hipFree(g_ctx.sh_partial);
+    (void)hipFree(g_ctx.sh_counters);
+    (void)hipFree(g_ctx.sh_pp);
+    (void)hipFree(g_ctx.sh_coef);
+    (void)hipFree(g_ctx.sh_ptrs);
+    (void)hipHostFree(g_ctx.host_coef);
+  }
+  HIPCHK(hipMalloc(&g_ctx.sh_partial, sizeof(double) * kRedBlocks * nshift));
+  HIPCHK(hipMalloc(&g_ctx.sh_counters, sizeof(unsigned) * nshift));
+  HIPCHK(hipMemset(g_ctx.sh_counters, 0, sizeof(unsigned) * nshift));
+  HIPCHK(hipMalloc(&g_ctx.sh_pp, sizeof(double) * nshift));
+  HIPCHK(hipMalloc(&g_ctx.sh_coef, sizeof(double) * 3 * nshift));
+  HIPCHK(hipMalloc(&g_ctx.sh_ptrs, sizeof(void*) * 2 * nshift));
+  HIPCHK(hipHostMalloc(&g_ctx.host_coef, sizeof(double) * (3 * nshift + 2 * nshift + nshift),
+                       hipHostMallocDefault));
+  g_ctx.sh_cap = nshift;
+  return DWF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-chirality Ls x Ls matrices (host, long double)
+// ---------------------------------------------------------------------------
+struct ChiMat {
+  int Ls = 0;
+  std::vector<long double> m[2];  // [chi][so * Ls + si]
+  explicit ChiMat(int L = 0) : Ls(L) {
+    m[0].assign((size_t)L * L, 0.0L);
+    m[1].assign((size_t)L * L, 0.0L);
+  }
+  long double& at(int chi, int so, int si) { return m[chi][(size_t)so * Ls + si]; }
+  long double at(int chi, int so, int si) const { return m[chi][(size_t)so * Ls + si]; }
+};
+
+static ChiMat cm_identity(int L) {
+  ChiMat r(L);
+  for (int c = 0; c < 2; ++c)
+    for (int s = 0; s < L; ++s) r.at(c, s, s) = 1.0L;
+  return r;
+}
+static ChiMat cm_mul(const ChiMat& a, const ChiMat& b) {
+  const int L = a.Ls;
+  ChiMat r(L);
+  for (int c = 0; c < 2; ++c)
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) {
+        long double s = 0;
+        for (int k = 0; k < L; ++k) s += a.at(c, i, k) * b.at(c, k, j);
+        r.at(c, i, j) = s;
+      }
+  return r;
+}
+static ChiMat cm_scale(const ChiMat& a, long double f) {
+  ChiMat r = a;
+  for (int c = 0; c < 2; ++c)
+    for (auto& v : r.m[c]) v *= f;
+  return r;
+}
+static ChiMat cm_transpose(const ChiMat& a) {
+  const int L = a.Ls;
+  ChiMat r(L);
+  for (int c = 0; c < 2; ++c)
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) r.at(c, i, j) = a.at(c, j, i);
+  return r;
+}
+// Gauss-Jordan with partial pivoting
+static bool cm_inverse(const ChiMat& a, ChiMat& out) {
+  const int L = a.Ls;
+  out = ChiMat(L);
+  for (int c = 0; c < 2; ++c) {
+    std::vector<long double> A(a.m[c]), B((size_t)L * L, 0.0L);
+    for (int i = 0; i < L; ++i) B[(size_t)i * L + i] = 1.0L;
+    for (int col = 0; col < L; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < L; ++r)
+        if (std::fabs(A[(size_t)r * L + col]) > std::fabs(A[(size_t)piv * L + col])) piv = r;
+      if (A[(size_t)piv * L + col] == 0.0L) return false;
+      if (piv != col)
+        for (int k = 0; k < L; ++k) {
+          std::swap(A[(size_t)piv * L + k], A[(size_t)col * L + k]);
+          std::swap(B[(size_t)piv * L + k], B[(size_t)col * L + k]);
+        }
+      const long double inv = 1.0L / A[(size_t)col * L + col];
+      for (int k = 0; k < L; ++k) {
+        A[(size_t)col * L + k] *= inv;
+        B[(size_t)col * L + k] *= inv;
+      }
+      for (int r = 0; r < L; ++r) {
+        if (r == col) continue;
+        const long double f = A[(size_t)r * L + col];
+        if (f == 0.0L) continue;
+        for (int k = 0; k < L; ++k) {
+          A[(size_t)r * L + k] -= f * A[(size_t)col * L + k];
+          B[(size_t)r * L + k] -= f * B[(size_t)col * L + k];
+        }
+      }
+    }
+    out.m[c] = B;
+  }
+  return true;
+}
+static bool cm_chirally_symmetric(const ChiMat& a) {
+  for (size_t i = 0; i < a.m[0].size(); ++i)
+    if (a.m[0][i] != a.m[1][i]) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// operator object
+// ---------------------------------------------------------------------------
+enum MatId {
+  M_PRE,        // Mp (site-local prologue when Mp is not spin-scalar)
+  M_E1,         // first hop of D
+  M_E2,         // second hop of D (with the -1)
+  M_DIT,        // Di^T
+  M_G1,         // first hop of Ddag: Di^T Mp^T k
+  M_G2,         // second hop of Ddag: -Mp^T k
+  M_DEO,        // epilogue of the Deo/Doe modes (k Mp when spin-scalar)
+  M_DEODAG,     // Mp^T k
+  M_DEE, M_DEET, M_DI,
+  M_G5, M_G5R,
+  M_COUNT
+};
+
+struct dwf_fopr_s {
+  int N[4];
+  int Ls;
+  int V, Vh;
+  int prec;
+  double mq, M0, alpha;
+  std::vector<double> b, c;
+  int bc[4];
+  std::string mode = "D";
+  bool mp_scalar = false;
+  size_t nelem = 0;         // reals per field
+  void* links = nullptr;
+  int* nbr = nullptr;
+  void* mats = nullptr;     // M_COUNT * 2 * Ls * Ls compute-type values
+  double* stage = nullptr;  // corelib staging (double), max(field, gauge)
+  size_t stage_bytes = 0;
+  void* work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool have_config = false;
+};
+
+static size_t cbytes(int prec) { return prec == P64 ? 8 : 4; }
+
+static const void* mat(const dwf_fopr_s* op, int id) {
+  return (const char*)op->mats + (size_t)id * 2 * op->Ls * op->Ls * cbytes(op->prec);
+}
+
+static int upload_mats(dwf_fopr_s* op, const std::vector<ChiMat>& ms) {
+  const int L = op->Ls;
+  const size_t per = 2 * (size_t)L * L;
+  if (op->prec == P64) {
+    std::vector<double> h(per * M_COUNT);
+    for (int id = 0; id < M_COUNT; ++id)
+      for (int c = 0; c < 2; ++c)
+        for (int i = 0; i < L * L; ++i)
+          h[id * per + c * L * L + i] = (double)(0.5L * ms[id].m[c][i]);
+    HIPCHK(hipMemcpy(op->mats, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h(per * M_COUNT);
+    for (int id = 0; id < M_COUNT; ++id)
+      for (int c = 0; c < 2; ++c)
+        for (int i = 0; i < L * L; ++i)
+          h[id * per + c * L * L + i] = (float)(0.5L * ms[id].m[c][i]);
+    HIPCHK(hipMemcpy(op->mats, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return DWF_OK;
+}
+
+// Builds the per-chirality matrices of the operator.  The 5th-dimension
+// structure is that of afopr_Domainwall_eo-tmpl.h: D_eo uses
+//   w'_s = b_s A_s(w_s) + c_s [ P- w_{s+1} (alpha, or -mq at s = Ls-1)
+//                             + P+ w_{s-1} (alpha, or -mq at s = 0) ]
+// (:765-828), D_ee the same with b -> b(4-M0)+1, c -> c(4-M0)-1 (:900-959);
+// A_s is alpha in the bulk and (alpha P+ + P-) / (P+ + alpha P-) at s = 0 /
+// Ls-1.  Chirality 0 is P+ (gamma_5 = +1), 1 is P-.
+static int build_mats(dwf_fopr_s* op) {
+  const int L = op->Ls;
+  const long double al = op->alpha, mq = op->mq, M0 = op->M0;
+  ChiMat Mp(L), Dee(L);
+  for (int s = 0; s < L; ++s) {
+    const long double Ap = (s == 0) ? al : (s == L - 1 ? 1.0L : al);
+    const long double Am = (s == 0) ? 1.0L : (s == L - 1 ? al : al);
+    const long double cdn = (s == 0) ? -mq : al;      // coefficient of P+ w_{s-1}
+    const long double cup = (s == L - 1) ? -mq : al;  // coefficient of P- w_{s+1}
+    const int sd = (s - 1 + L) % L, su = (s + 1) % L;
+    const long double b = op->b[s], c = op->c[s];
+    Mp.at(0, s, s) += b * Ap;
+    Mp.at(0, s, sd) += c * cdn;
+    Mp.at(1, s, s) += b * Am;
+    Mp.at(1, s, su) += c * cup;
+    const long double F1 = b * (4.0L - M0) + 1.0L, F2 = c * (4.0L - M0) - 1.0L;
+    Dee.at(0, s, s) += F1 * Ap;
+    Dee.at(0, s, sd) += F2 * cdn;
+    Dee.at(1, s, s) += F1 * Am;
+    Dee.at(1, s, su) += F2 * cup;
+  }
+  ChiMat Di;
+  if (!cm_inverse(Dee, Di)) return fail(DWF_ERR_INVALID_ARG, "D_ee is singular");
+  const long double kappa = 1.0L / (8.0L - 2.0L * M0);
+  const long double k = -kappa * (4.0L - M0);
+  op->mp_scalar = cm_chirally_symmetric(Mp);
+  const ChiMat I = cm_identity(L);
+  const ChiMat MpT = cm_transpose(Mp), DiT = cm_transpose(Di);
+  std::vector<ChiMat> ms(M_COUNT, ChiMat(L));
+  ms[M_PRE] = Mp;
+  if (op->mp_scalar) {
+    ms[M_E1] = cm_scale(cm_mul(Di, Mp), k);
+    ms[M_E2] = cm_scale(cm_mul(Di, Mp), -k);
+    ms[M_DEO] = cm_scale(Mp, k);
+  } else {
+    ms[M_E1] = cm_scale(cm_mul(Mp, Di), k);
+    ms[M_E2] = cm_scale(Di, -k);
+    ms[M_DEO] = cm_scale(I, k);
+  }
+  ms[M_DIT] = DiT;
+  ms[M_G1] = cm_scale(cm_mul(DiT, MpT), k);
+  ms[M_G2] = cm_scale(MpT, -k);
+  ms[M_DEODAG] = cm_scale(MpT, k);
+  ms[M_DEE] = Dee;
+  ms[M_DEET] = cm_transpose(Dee);
+  ms[M_DI] = Di;
+  ChiMat G5(L), G5R(L);
+  for (int s = 0; s < L; ++s) {
+    G5.at(0, s, s) = 1.0L;
+    G5.at(1, s, s) = -1.0L;
+    G5R.at(0, s, L - 1 - s) = 1.0L;
+    G5R.at(1, s, L - 1 - s) = -1.0L;
+  }
+  ms[M_G5] = G5;
+  ms[M_G5R] = G5R;
+  return upload_mats(op, ms);
+}
+
+static int build_nbr(dwf_fopr_s* op) {
+  const int Nx = op->N[0], Ny = op->N[1], Nz = op->N[2], Nt = op->N[3];
+  const int Nx2 = Nx / 2, Vh = op->Vh;
+  std::vector<int> h((size_t)2 * 8 * Vh);
+  for (int p = 0; p < 2; ++p)
+    for (int s2 = 0; s2 < Vh; ++s2) {
+      int r = s2 / Nx2;
+      const int x2 = s2 - r * Nx2;
+      const int y = r % Ny;
+      r /= Ny;
+      const int z = r % Nz;
+      const int t = r / Nz;
+      const int x = 2 * x2 + ((y + z + t + p) & 1);
+      const int xc[4] = {x, y, z, t};
+      for (int mu = 0; mu < 4; ++mu)
+        for (int fb = 0; fb < 2; ++fb) {
+          int n[4] = {xc[0], xc[1], xc[2], xc[3]};
+          n[mu] = (n[mu] + (fb == 0 ? 1 : -1) + op->N[mu]) % op->N[mu];
+          h[((size_t)p * 8 + 2 * mu + fb) * Vh + s2] =
+              n[0] / 2 + Nx2 * (n[1] + Ny * (n[2] + Nz * n[3]));
+        }
+    }
+  HIPCHK(hipMemcpy(op->nbr, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  return DWF_OK;
+}
+
+static int get_work(dwf_fopr_s* op, int i, void** out) {
+  if (!op->work[i]) HIPCHK(hipMalloc(&op->work[i], op->nelem * prec_bytes(op->prec)));
+  *out = op->work[i];
+  return DWF_OK;
+}
+
+// launch helpers ------------------------------------------------------------
+static int run_site(dwf_fopr_s* op, void* out, const void* in, int mid, void* out2 = nullptr,
+                    int fid = -1) {
+  Epi e;
+  e.E = mat(op, mid);
+  if (out2) {
+    e.out2 = out2;
+    e.F = mat(op, fid);
+  }
+  HIPCHK(launch_dwf(op->prec, K_SITE, out, in, op->links, op->nbr, 0, op->Vh, op->Ls, e,
+                    g_ctx.stream));
+  return DWF_OK;
+}
+static int run_hop(dwf_fopr_s* op, bool dag, int parity, void* out, const void* in, int mid,
+                   const void* X = nullptr, double xcoef = 0.0, void* out2 = nullptr,
+                   int fid = -1) {
+  Epi e;
+  e.E = mat(op, mid);
+  e.X = X;
+  e.xcoef = xcoef;
+  if (out2) {
+    e.out2 = out2;
+    e.F = mat(op, fid);
+  }
+  HIPCHK(launch_dwf(op->prec, dag ? K_HOPDAG : K_HOP, out, in, op->links, op->nbr, parity,
+                    op->Vh, op->Ls, e, g_ctx.stream));
+  return DWF_OK;
+}
+
+#define RC(expr)                  \
+  do {                            \
+    int rc__ = (expr);            \
+    if (rc__ != DWF_OK) return rc__; \
+  } while (0)
+
+// D v = w - Di k Hop_eo Mp Di k Hop_oe Mp w; optional v2 = Di^T v (fused)
+static int op_D(dwf_fopr_s* op, void* v, const void* w, void* v2 = nullptr) {
+  void *t0, *t1;
+  RC(get_work(op, 0, &t0));
+  RC(get_work(op, 1, &t1));
+  const void* src = w;
+  if (!op->mp_scalar) {
+    RC(run_site(op, t0, w, M_PRE));
+    src = t0;
+  }
+  RC(run_hop(op, false, 1, t1, src, M_E1));
+  RC(run_hop(op, false, 0, v, t1, M_E2, w, 1.0, v2, M_DIT));
+  return DWF_OK;
+}
+// Ddag v = w - Mp^T k Hop+_eo Di^T Mp^T k Hop+_oe Di^T w; pre = Di^T w if known
+static int op_Ddag(dwf_fopr_s* op, void* v, const void* w, const void* pre = nullptr) {
+  void *t0, *t1;
+  RC(get_work(op, 0, &t0));
+  RC(get_work(op, 1, &t1));
+  if (!pre) {
+    RC(run_site(op, t0, w, M_DIT));
+    pre = t0;
+  }
+  RC(run_hop(op, true, 1, t1, pre, M_G1));
+  RC(run_hop(op, true, 0, v, t1, M_G2, w, 1.0));
+  return DWF_OK;
+}
+static int op_DdagD(dwf_fopr_s* op, void* v, const void* w) {
+  void *u, *u2;
+  RC(get_work(op, 2, &u));
+  RC(get_work(op, 3, &u2));
+  RC(op_D(op, u, w, u2));
+  RC(op_Ddag(op, v, u, u2));
+  return DWF_OK;
+}
+static int op_DDdag(dwf_fopr_s* op, void* v, const void* w) {
+  void* u;
+  RC(get_work(op, 2, &u));
+  RC(op_Ddag(op, u, w));
+  RC(op_D(op, v, u));
+  return DWF_OK;
+}
+static int op_H(dwf_fopr_s* op, void* v, const void* w) {
+  void* u;
+  RC(get_work(op, 2, &u));
+  RC(op_D(op, u, w));
+  RC(run_site(op, v, u, M_G5R));
+  return DWF_OK;
+}
+static int op_Hdag(dwf_fopr_s* op, void* v, const void* w) {
+  void *u, *u2;
+  RC(get_work(op, 2, &u));
+  RC(get_work(op, 3, &u2));
+  RC(run_site(op, u, w, M_G5R, u2, M_DIT));
+  RC(op_Ddag(op, v, u, u2));
+  return DWF_OK;
+}
+// D_eo / D_oe (mult modes "Deo" = output even, "Doe" = output odd)
+static int op_Deo(dwf_fopr_s* op, void* v, const void* w, int parity) {
+  const void* src = w;
+  if (!op->mp_scalar) {
+    void* t0;
+    RC(get_work(op, 0, &t0));
+    RC(run_site(op, t0, w, M_PRE));
+    src = t0;
+  }
+  RC(run_hop(op, false, parity, v, src, M_DEO));
+  return DWF_OK;
+}
+
+static bool hop_last(const std::string& m) {
+  return m == "D" || m == "Ddag" || m == "DdagD" || m == "DDdag" || m == "H" || m == "Hdag" ||
+         m == "Deo" || m == "Doe";
+}
+
+static int mult_impl(dwf_fopr_s* op, void* v, const void* w, const std::string& m, bool dag) {
+  if (!op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  if (!v || !w) return fail(DWF_ERR_INVALID_ARG, "null field");
+  // kernels that gather neighbours must not write their own input: an aliased
+  // input is first copied to a buffer of its own
+  const void* src = w;
+  if (v == w && hop_last(m)) {
+    void* tmp;
+    RC(get_work(op, 4, &tmp));
+    HIPCHK(launch_copy(op->prec, tmp, w, op->nelem, g_ctx.stream));
+    src = tmp;
+  }
+  int rc = DWF_OK;
+  if (!dag) {
+    if (m == "D") rc = op_D(op, v, src);
+    else if (m == "Ddag") rc = op_Ddag(op, v, src);
+    else if (m == "DdagD") rc = op_DdagD(op, v, src);
+    else if (m == "DDdag") rc = op_DDdag(op, v, src);
+    else if (m == "H") rc = op_H(op, v, src);
+    else if (m == "Hdag") rc = op_Hdag(op, v, src);
+    else if (m == "Deo") rc = op_Deo(op, v, src, 0);
+    else if (m == "Doe") rc = op_Deo(op, v, src, 1);
+    else if (m == "Dee" || m == "Doo") rc = run_site(op, v, src, M_DEE);
+    else if (m == "Dee_inv" || m == "Doo_inv") rc = run_site(op, v, src, M_DI);
+    else rc = fail(DWF_ERR_UNKNOWN_MODE, "mode undefined: " + m);
+  } else {
+    if (m == "D") rc = op_Ddag(op, v, src);
+    else if (m == "Ddag") rc = op_D(op, v, src);
+    else if (m == "DdagD") rc = op_DdagD(op, v, src);
+    else if (m == "DDdag") rc = op_DDdag(op, v, src);
+    else if (m == "H") rc = op_Hdag(op, v, src);
+    else if (m == "Hdag") rc = op_H(op, v, src);
+    else if (m == "Deo") rc = run_hop(op, true, 1, v, src, M_DEODAG);   // Ddag_eo(v, w, 1)
+    else if (m == "Doe") rc = run_hop(op, true, 0, v, src, M_DEODAG);   // Ddag_eo(v, w, 0)
+    else if (m == "Dee" || m == "Doo") rc = run_site(op, v, src, M_DEET);
+    else if (m == "Dee_inv" || m == "Doo_inv") rc = run_site(op, v, src, M_DIT);
+    else rc = fail(DWF_ERR_UNKNOWN_MODE, "mode undefined: " + m);
+  }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* dwf_last_error(void) { return g_err.c_str(); }
+const char* dwf_version(void) { return "dwfhip 0.1 (gfx950)"; }
+void* dwf_stream(void) {
+  if (ctx_init() != DWF_OK) return nullptr;
+  return (void*)g_ctx.stream;
+}
+int dwf_synchronize(void) {
+  CTX();
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_fopr_create(const dwf_fopr_params* p, dwf_fopr* out) {
+  CTX();
+  if (!p || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  for (int mu = 0; mu < 4; ++mu)
+    if (p->lattice_size[mu] < 2 || (p->lattice_size[mu] & 1))
+      return fail(DWF_ERR_INVALID_ARG, "lattice extents must be even and >= 2");
+  const int Ls = p->extent_of_5th_dimension;
+  if (Ls < 2 || Ls > 48) return fail(DWF_ERR_UNSUPPORTED, "extent_of_5th_dimension must be in [2, 48]");
+  if (p->precision < DWF_F64 || p->precision > DWF_BF16)
+    return fail(DWF_ERR_INVALID_ARG, "unknown precision");
+  if (p->domain_wall_height == 4.0) return fail(DWF_ERR_INVALID_ARG, "M0 = 4 is singular");
+  dwf_fopr_s* op = new dwf_fopr_s;
+  for (int mu = 0; mu < 4; ++mu) {
+    op->N[mu] = p->lattice_size[mu];
+    op->bc[mu] = p->boundary_condition[mu];
+  }
+  op->Ls = Ls;
+  op->V = op->N[0] * op->N[1] * op->N[2] * op->N[3];
+  op->Vh = op->V / 2;
+  op->prec = p->precision;
+  op->mq = p->quark_mass;
+  op->M0 = p->domain_wall_height;
+  op->alpha = p->parameter_alpha == 0.0 ? 1.0 : p->parameter_alpha;
+  op->b.resize(Ls);
+  op->c.resize(Ls);
+  for (int s = 0; s < Ls; ++s) {
+    op->b[s] = p->coefficient_b ? p->coefficient_b[s] : p->b_uniform;
+    op->c[s] = p->coefficient_c ? p->coefficient_c[s] : p->c_uniform;
+  }
+  op->nelem = (size_t)op->Vh * 24 * Ls;
+  auto cleanup = [&](int rc) {
+    dwf_fopr_destroy(op);
+    return rc;
+  };
+  if (hipMalloc(&op->links, (size_t)2 * 8 * 18 * op->Vh * prec_bytes(op->prec)) != hipSuccess ||
+      hipMalloc((void**)&op->nbr, (size_t)2 * 8 * op->Vh * sizeof(int)) != hipSuccess ||
+      hipMalloc(&op->mats, (size_t)M_COUNT * 2 * Ls * Ls * cbytes(op->prec)) != hipSuccess)
+    return cleanup(fail(DWF_ERR_HIP, "device allocation failed"));
+  op->stage_bytes = std::max(op->nelem, (size_t)op->V * 4 * 18) * sizeof(double);
+  if (hipMalloc(&op->stage, op->stage_bytes) != hipSuccess)
+    return cleanup(fail(DWF_ERR_HIP, "device allocation failed"));
+  int rc = build_nbr(op);
+  if (rc != DWF_OK) return cleanup(rc);
+  rc = build_mats(op);
+  if (rc != DWF_OK) return cleanup(rc);
+  *out = op;
+  return DWF_OK;
+}
+
+int dwf_fopr_destroy(dwf_fopr op) {
+  if (!op) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  if (op->links) (void)hipFree(op->links);
+  if (op->nbr) (void)hipFree(op->nbr);
+  if (op->mats) (void)hipFree(op->mats);
+  if (op->stage) (void)hipFree(op->stage);
+  for (auto& w : op->work)
+    if (w) (void)hipFree(w);
+  delete op;
+  return DWF_OK;
+}
+
+int dwf_fopr_set_config_device(dwf_fopr op, const double* U) {
+  CTX();
+  if (!op || !U) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_pack_links(op->prec, op->links, U, op->N[0], op->N[1], op->N[2], op->N[3],
+                           op->bc[0], op->bc[1], op->bc[2], op->bc[3], g_ctx.stream));
+  op->have_config = true;
+  return DWF_OK;
+}
+
+int dwf_fopr_set_config(dwf_fopr op, const double* U_host) {
+  CTX();
+  if (!op || !U_host) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  const size_t bytes = (size_t)op->V * 4 * 18 * sizeof(double);
+  HIPCHK(hipMemcpyAsync(op->stage, U_host, bytes, hipMemcpyHostToDevice, g_ctx.stream));
+  RC(dwf_fopr_set_config_device(op, op->stage));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_fopr_set_mode(dwf_fopr op, const char* mode) {
+  if (!op || !mode) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  static const char* known[] = {"D", "Ddag", "DdagD", "DDdag", "H", "Hdag", "Deo", "Doe",
+                                "Dee", "Doo", "Dee_inv", "Doo_inv"};
+  for (const char* k : known)
+    if (std::strcmp(k, mode) == 0) {
+      op->mode = mode;
+      return DWF_OK;
+    }
+  return fail(DWF_ERR_UNKNOWN_MODE, std::string("mode undefined: ") + mode);
+}
+const char* dwf_fopr_get_mode(dwf_fopr op) { return op ? op->mode.c_str() : ""; }
+
+int dwf_fopr_mult(dwf_fopr op, void* v, const void* w) {
+  CTX();
+  if (!op) return fail(DWF_ERR_INVALID_ARG, "null operator");
+  return mult_impl(op, v, w, op->mode, false);
+}
+int dwf_fopr_mult_dag(dwf_fopr op, void* v, const void* w) {
+  CTX();
+  if (!op) return fail(DWF_ERR_INVALID_ARG, "null operator");
+  return mult_impl(op, v, w, op->mode, true);
+}
+int dwf_fopr_mult_mode(dwf_fopr op, void* v, const void* w, const char* mode) {
+  CTX();
+  if (!op || !mode) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  std::string m(mode);
+  if (m != "Deo" && m != "Doe" && m != "Dee" && m != "Doo" && m != "Dee_inv" && m != "Doo_inv")
+    return fail(DWF_ERR_UNKNOWN_MODE, "mode undefined for mult(v, w, mode): " + m);
+  return mult_impl(op, v, w, m, false);
+}
+int dwf_fopr_mult_dag_mode(dwf_fopr op, void* v, const void* w, const char* mode) {
+  CTX();
+  if (!op || !mode) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  std::string m(mode);
+  if (m != "Deo" && m != "Doe" && m != "Dee" && m != "Doo" && m != "Dee_inv" && m != "Doo_inv")
+    return fail(DWF_ERR_UNKNOWN_MODE, "mode undefined for mult_dag(v, w, mode): " + m);
+  return mult_impl(op, v, w, m, true);
+}
+int dwf_fopr_mult_gm5(dwf_fopr op, void* v, const void* w) {
+  CTX();
+  if (!op || !v || !w) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  return run_site(op, v, w, M_G5);
+}
+int dwf_fopr_mult_gm5R(dwf_fopr op, void* v, const void* w) {
+  CTX();
+  if (!op || !v || !w) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (v == w) return fail(DWF_ERR_INVALID_ARG, "mult_gm5R needs distinct fields");
+  return run_site(op, v, w, M_G5R);
+}
+
+int dwf_fopr_field_nin(dwf_fopr op) { return op ? 24 * op->Ls : 0; }
+int dwf_fopr_field_nvol(dwf_fopr op) { return op ? op->Vh : 0; }
+int dwf_fopr_field_nex(dwf_fopr op) { return op ? 1 : 0; }
+size_t dwf_fopr_field_bytes(dwf_fopr op) { return op ? op->nelem * prec_bytes(op->prec) : 0; }
+int dwf_fopr_precision(dwf_fopr op) { return op ? op->prec : -1; }
+
+double dwf_fopr_flop_count(dwf_fopr op, const char* mode) {
+  // afopr_Domainwall_5din_eo-tmpl.h:1196-1250 (Dirac representation)
+  if (!op || !mode) return 0.0;
+  const double vsite = op->V, Ns = op->Ls, Nc = 3, Nd = 4;
+  const double hop = Ns * Nc * Nd * (6 * (4 * Nc + 2) + 2 * (4 * Nc + 1)) * vsite;
+  const double pre = Ns * Nc * Nd * 14 * vsite;
+  const double lu = Nc * Nd * (2 + (Ns - 1) * 26) * vsite;
+  const double axpy = Ns * 2 * Nc * Nd * vsite;
+  const double deo = hop + pre;
+  std::string m(mode);
+  if (m == "D" || m == "Ddag") return deo + lu + 0.5 * axpy;
+  if (m == "DdagD") return 2 * deo + 2 * lu + axpy;
+  if (m == "Dee_inv" || m == "Doo_inv") return 0.5 * lu;
+  if (m == "Dee" || m == "Doo") return 0.5 * pre;
+  if (m == "Deo" || m == "Doe") return 0.5 * deo;
+  return 0.0;
+}
+
+int dwf_fopr_convert_device(dwf_fopr op, void* v, const double* w) {
+  CTX();
+  if (!op || !v || !w) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_to_device_layout(op->prec, v, w, op->Vh, op->Ls, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_fopr_reverse_device(dwf_fopr op, double* v, const void* w) {
+  CTX();
+  if (!op || !v || !w) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_to_corelib_layout(op->prec, v, w, op->Vh, op->Ls, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_fopr_convert(dwf_fopr op, void* v, const double* w_host) {
+  CTX();
+  if (!op || !v || !w_host) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(op->stage, w_host, op->nelem * sizeof(double), hipMemcpyHostToDevice,
+                        g_ctx.stream));
+  RC(dwf_fopr_convert_device(op, v, op->stage));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_fopr_reverse(dwf_fopr op, double* v_host, const void* w) {
+  CTX();
+  if (!op || !v_host || !w) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  RC(dwf_fopr_reverse_device(op, op->stage, w));
+  HIPCHK(hipMemcpyAsync(v_host, op->stage, op->nelem * sizeof(double), hipMemcpyDeviceToHost,
+                        g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+
+// ---- fields ---------------------------------------------------------------
+int dwf_field_alloc(dwf_fopr op, void** f) {
+  CTX();
+  if (!op || !f) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipMalloc(f, op->nelem * prec_bytes(op->prec)));
+  HIPCHK(hipMemsetAsync(*f, 0, op->nelem * prec_bytes(op->prec), g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_free(void* f) {
+  if (!f) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  HIPCHK(hipFree(f));
+  return DWF_OK;
+}
+int dwf_field_set_zero(dwf_fopr op, void* x) {
+  CTX();
+  HIPCHK(launch_set_zero(op->prec, x, op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_copy(dwf_fopr op, void* y, const void* x) {
+  CTX();
+  HIPCHK(launch_copy(op->prec, y, x, op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_axpy(dwf_fopr op, void* y, double a, const void* x) {
+  CTX();
+  HIPCHK(launch_axpy(op->prec, y, a, x, op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_aypx(dwf_fopr op, double a, void* y, const void* x) {
+  CTX();
+  HIPCHK(launch_aypx(op->prec, a, y, x, op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_scal(dwf_fopr op, void* x, double a) {
+  CTX();
+  HIPCHK(launch_scal(op->prec, x, a, op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_norm2(dwf_fopr op, const void* x, double* r) {
+  CTX();
+  HIPCHK(launch_norm2(op->prec, x, op->nelem, g_ctx.red, 0, g_ctx.stream));
+  return fetch(1, r);
+}
+int dwf_field_dot(dwf_fopr op, const void* x, const void* y, double* r) {
+  CTX();
+  HIPCHK(launch_dot(op->prec, x, y, op->nelem, g_ctx.red, 0, g_ctx.stream));
+  return fetch(1, r);
+}
+int dwf_field_dotc(dwf_fopr op, const void* x, const void* y, double* re, double* im) {
+  CTX();
+  HIPCHK(launch_dotc_ls(op->prec, x, y, op->nelem, op->Ls, g_ctx.red, 0, g_ctx.stream));
+  double v[2];
+  RC(fetch(2, v));
+  *re = v[0];
+  *im = v[1];
+  return DWF_OK;
+}
+int dwf_field_convert_precision(dwf_fopr op_to, void* v, dwf_fopr op_from, const void* w) {
+  CTX();
+  if (!op_to || !op_from || op_to->nelem != op_from->nelem)
+    return fail(DWF_ERR_INVALID_ARG, "operators of different lattices");
+  HIPCHK(launch_convert(op_to->prec, v, op_from->prec, w, op_to->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+
+// ---- solvers ----------------------------------------------------------------
+struct dwf_solver_s {
+  int kind = 0;  // 0 CG, 1 Richardson(mixed)
+  dwf_fopr op = nullptr;
+  dwf_fopr op_f = nullptr;
+  int niter = 0;
+  double stop = 0;
+  int guess = DWF_GUESS_RHS;
+  int niter_inner = 0;
+  double stop_inner = 0;
+  void* f[4] = {nullptr, nullptr, nullptr, nullptr};   // work in op precision
+  void* g[2] = {nullptr, nullptr};                     // work in op_f precision
+  dwf_solver_s* inner = nullptr;
+  int inner_iters = 0;
+};
+
+static int solver_field(dwf_fopr op, void** slot) {
+  if (!*slot) HIPCHK(hipMalloc(slot, op->nelem * prec_bytes(op->prec)));
+  return DWF_OK;
+}
+
+// ASolver_CG::solve (asolver_CG-tmpl.h:118-234)
+static int cg_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out, double* diff_out) {
+  dwf_fopr op = s->op;
+  for (int i = 0; i < 4; ++i) RC(solver_field(op, &s->f[i]));
+  void *x = s->f[0], *r = s->f[1], *p = s->f[2], *ss = s->f[3];
+  const int pr = op->prec;
+  const size_t n = op->nelem;
+  hipStream_t st = g_ctx.stream;
+  double sr;
+  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &sr));
+  if (!(sr > 0)) {
+    HIPCHK(launch_set_zero(pr, xq, n, st));
+    *nconv_out = 0;
+    *diff_out = 0;
+    return DWF_OK;
+  }
+  const double snorm = 1.0 / sr;
+  double rr, rrp;
+  if (s->guess == DWF_GUESS_RHS) {
+    HIPCHK(launch_copy(pr, r, b, n, st));
+    HIPCHK(launch_copy(pr, x, b, n, st));
+    RC(mult_impl(op, ss, x, op->mode, false));
+    HIPCHK(launch_axpy(pr, r, -1.0, ss, n, st));
+  } else if (s->guess == DWF_GUESS_ZERO) {
+    HIPCHK(launch_copy(pr, r, b, n, st));
+    HIPCHK(launch_set_zero(pr, x, n, st));
+  } else {
+    return fail(DWF_ERR_UNSUPPORTED, "InitialGuess::GIVEN is not supported by ASolver_CG");
+  }
+  HIPCHK(launch_copy(pr, p, r, n, st));
+  HIPCHK(launch_norm2(pr, r, n, g_ctx.red, 0, st));
+  RC(fetch(1, &rr));
+  rrp = rr;
+  int nconv = -1;
+  for (int iter = 0; iter < s->niter; ++iter) {
+    RC(mult_impl(op, ss, p, op->mode, false));
+    double pap;
+    HIPCHK(launch_dot(pr, ss, p, n, g_ctx.red, 0, st));
+    RC(fetch(1, &pap));
+    const double cr = rrp / pap;
+    HIPCHK(launch_cg_xr(pr, x, r, p, ss, cr, n, g_ctx.red, 0, st));
+    RC(fetch(1, &rr));
+    const double bk = rr / rrp;
+    HIPCHK(launch_aypx(pr, bk, p, r, n, st));
+    rrp = rr;
+    if (rr * snorm < s->stop) {
+      nconv = iter + 1;
+      break;
+    }
+  }
+  HIPCHK(launch_copy(pr, xq, x, n, st));
+  RC(mult_impl(op, ss, xq, op->mode, false));
+  double diff2;
+  HIPCHK(launch_sub_norm2(pr, ss, ss, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &diff2));
+  *nconv_out = nconv;
+  *diff_out = std::sqrt(diff2 * snorm);
+  s->inner_iters = nconv > 0 ? nconv : s->niter;
+  return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "ASolver_CG: not converged") : DWF_OK;
+}
+
+// ASolver_Richardson::solve with APrecond_Mixedprec
+// (asolver_Richardson-tmpl.h:131-213, aprecond_Mixedprec-tmpl.h:55-108)
+static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
+                            double* diff_out) {
+  dwf_fopr op = s->op, opf = s->op_f;
+  RC(solver_field(op, &s->f[0]));
+  RC(solver_field(op, &s->f[1]));
+  RC(solver_field(opf, &s->g[0]));
+  RC(solver_field(opf, &s->g[1]));
+  void *r = s->f[0], *sv = s->f[1];
+  void *rf = s->g[0], *sf = s->g[1];
+  const int pr = op->prec;
+  const size_t n = op->nelem;
+  hipStream_t st = g_ctx.stream;
+  s->inner_iters = 0;
+  double bnorm2;
+  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &bnorm2));
+  if (!(bnorm2 > 0.0)) {
+    HIPCHK(launch_set_zero(pr, xq, n, st));
+    *nconv_out = 0;
+    *diff_out = 0.0;
+    return DWF_OK;
+  }
+  const double bnorm2_inv = 1.0 / bnorm2;
+  HIPCHK(launch_copy(pr, r, b, n, st));
+  HIPCHK(launch_scal(pr, r, -1.0, n, st));
+  HIPCHK(launch_set_zero(pr, xq, n, st));
+  int nconv = -1;
+  double r2 = bnorm2;
+  for (int iter = 0; iter < s->niter; ++iter) {
+    if (r2 < bnorm2 * s->stop) {
+      nconv = iter;
+      break;
+    }
+    const double normalization = std::sqrt(r2);
+    HIPCHK(launch_scal(pr, r, 1.0 / normalization, n, st));
+    HIPCHK(launch_convert(opf->prec, rf, pr, r, n, st));
+    int inc = 0;
+    double idiff = 0;
+    int rc = cg_solve(s->inner, sf, rf, &inc, &idiff);
+    if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
+    s->inner_iters += s->inner->inner_iters;
+    HIPCHK(launch_convert(pr, sv, opf->prec, sf, n, st));
+    HIPCHK(launch_axpy(pr, xq, -normalization, sv, n, st));
+    RC(mult_impl(op, r, xq, op->mode, false));
+    HIPCHK(launch_sub_norm2(pr, r, r, b, n, g_ctx.red, 0, st));
+    RC(fetch(1, &r2));
+  }
+  RC(mult_impl(op, r, xq, op->mode, false));
+  double diff2;
+  HIPCHK(launch_sub_norm2(pr, r, r, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &diff2));
+  *nconv_out = nconv;
+  *diff_out = std::sqrt(diff2 * bnorm2_inv);
+  return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "ASolver_Richardson: not converged") : DWF_OK;
+}
+
+int dwf_solver_create_cg(dwf_fopr op, int max_iter, double stop_cond, int guess,
+                         dwf_solver* out) {
+  CTX();
+  if (!op || !out || max_iter < 0) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (guess != DWF_GUESS_RHS && guess != DWF_GUESS_ZERO && guess != DWF_GUESS_GIVEN)
+    return fail(DWF_ERR_INVALID_ARG, "unknown initial guess mode");
+  dwf_solver_s* s = new dwf_solver_s;
+  s->kind = 0;
+  s->op = op;
+  s->niter = max_iter;
+  s->stop = stop_cond;
+  s->guess = guess;
+  *out = s;
+  return DWF_OK;
+}
+
+int dwf_solver_create_richardson(dwf_fopr op_d, dwf_fopr op_f, int max_iter, double stop_cond,
+                                 int max_iter_inner, double stop_cond_inner, dwf_solver* out) {
+  CTX();
+  if (!op_d || !op_f || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (op_d->nelem != op_f->nelem) return fail(DWF_ERR_INVALID_ARG, "operators differ in size");
+  dwf_solver_s* s = new dwf_solver_s;
+  s->kind = 1;
+  s->op = op_d;
+  s->op_f = op_f;
+  s->niter = max_iter;
+  s->stop = stop_cond;
+  s->niter_inner = max_iter_inner;
+  s->stop_inner = stop_cond_inner;
+  s->inner = new dwf_solver_s;
+  s->inner->kind = 0;
+  s->inner->op = op_f;
+  s->inner->niter = max_iter_inner;
+  s->inner->stop = stop_cond_inner;
+  s->inner->guess = DWF_GUESS_ZERO;
+  *out = s;
+  return DWF_OK;
+}
+
+int dwf_solver_solve(dwf_solver s, void* x, const void* b, int* nconv, double* diff) {
+  CTX();
+  if (!s || !x || !b || !nconv || !diff) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (!s->op->have_config || (s->op_f && !s->op_f->have_config))
+    return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  if (s->kind == 1) {
+    if (s->op_f->mode != s->op->mode) s->op_f->mode = s->op->mode;
+    return richardson_solve(s, x, b, nconv, diff);
+  }
+  return cg_solve(s, x, b, nconv, diff);
+}
+
+int dwf_solver_inner_iterations(dwf_solver s) { return s ? s->inner_iters : 0; }
+
+int dwf_solver_destroy(dwf_solver s) {
+  if (!s) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  for (auto& f : s->f)
+    if (f) (void)hipFree(f);
+  for (auto& g : s->g)
+    if (g) (void)hipFree(g);
+  if (s->inner) dwf_solver_destroy(s->inner);
+  delete s;
+  return DWF_OK;
+}
+
+// AShiftsolver_CG::solve (ashiftsolver_CG-tmpl.h:89-329), FP64
+int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int nshift,
+                             const double* sigma, void** xq, const void* b, int* nconv_out,
+                             double* diff_out) {
+  CTX();
+  if (!op || !sigma || !xq || !b || nshift < 1) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (op->prec != P64) return fail(DWF_ERR_UNSUPPORTED, "multishift CG runs on the FP64 operator");
+  if (!op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  RC(ensure_shift_scratch(nshift));
+  const size_t n = op->nelem;
+  const size_t bytes = n * sizeof(double);
+  hipStream_t st = g_ctx.stream;
+  std::vector<void*> p(nshift, nullptr);
+  void *s = nullptr, *r = nullptr;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (auto& q : p)
+      if (q) (void)hipFree(q);
+    if (s) (void)hipFree(s);
+    if (r) (void)hipFree(r);
+  };
+  for (int i = 0; i < nshift; ++i)
+    if (hipMalloc(&p[i], bytes) != hipSuccess) {
+      release();
+      return fail(DWF_ERR_HIP, "device allocation failed");
+    }
+  if (hipMalloc(&s, bytes) != hipSuccess || hipMalloc(&r, bytes) != hipSuccess) {
+    release();
+    return fail(DWF_ERR_HIP, "device allocation failed");
+  }
+  int rc = DWF_OK;
+  auto body = [&]() -> int {
+    double bn;
+    HIPCHK(launch_norm2(P64, b, n, g_ctx.red, 0, st));
+    RC(fetch(1, &bn));
+    if (!(bn > 0)) return fail(DWF_ERR_INVALID_ARG, "zero source");
+    const double snorm = 1.0 / bn;
+    std::vector<double> zeta1(nshift, 1.0), zeta2(nshift, 1.0), csh2(nshift), pp(nshift, 0.0);
+    for (int i = 0; i < nshift; ++i) csh2[i] = sigma[i] - sigma[0];
+    const double sigma0 = sigma[0];
+    int Nshift2 = nshift;
+    HIPCHK(launch_copy(P64, s, b, n, st));
+    HIPCHK(launch_copy(P64, r, b, n, st));
+    for (int i = 0; i < nshift; ++i) {
+      HIPCHK(launch_copy(P64, p[i], s, n, st));
+      HIPCHK(launch_set_zero(P64, xq[i], n, st));
+    }
+    // device pointer arrays for the batched shift update
+    void** hp = (void**)(g_ctx.host_coef + 3 * nshift);
+    for (int i = 0; i < nshift; ++i) {
+      hp[i] = xq[i];
+      hp[nshift + i] = p[i];
+    }
+    HIPCHK(hipMemcpyAsync(g_ctx.sh_ptrs, hp, sizeof(void*) * 2 * nshift, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double rr;
+    HIPCHK(launch_norm2(P64, r, n, g_ctx.red, 0, st));
+    RC(fetch(1, &rr));
+    double alpha_p = 0.0, beta_p = 1.0;
+    int Nconv2 = -1;
+    bool converged = false;
+    for (int iter = 0; iter < max_iter; ++iter) {
+      RC(mult_impl(op, s, p[0], op->mode, false));
+      double pa_p;
+      HIPCHK(launch_axpy_dot(P64, s, sigma0, p[0], n, g_ctx.red, 0, st));
+      RC(fetch(1, &pa_p));
+      const double rr_p = rr;
+      const double beta = -rr_p / pa_p;
+      HIPCHK(launch_cg_xr(P64, xq[0], r, p[0], s, -beta, n, g_ctx.red, 0, st));
+      RC(fetch(1, &rr));
+      const double alpha = rr / rr_p;
+      HIPCHK(launch_aypx(P64, alpha, p[0], r, n, st));
+      pp[0] = rr;
+      const double alpha_h = 1.0 + alpha_p * beta / beta_p;
+      std::vector<double> znew(nshift, 0.0);
+      double* hc = g_ctx.host_coef;
+      for (int ish = 1; ish < Nshift2; ++ish) {
+        double zeta = (alpha_h - csh2[ish] * beta) / zeta1[ish] + (1.0 - alpha_h) / zeta2[ish];
+        zeta = 1.0 / zeta;
+        const double zr = zeta / zeta1[ish];
+        const double beta_s = beta * zr;
+        const double alpha_s = alpha * zr * zr;
+        hc[3 * ish] = -beta_s;
+        hc[3 * ish + 1] = alpha_s;
+        hc[3 * ish + 2] = zeta;
+        znew[ish] = zeta;
+      }
+      if (Nshift2 > 1) {
+        HIPCHK(hipMemcpyAsync(g_ctx.sh_coef, hc, sizeof(double) * 3 * Nshift2,
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(launch_shift_update(g_ctx.sh_ptrs, g_ctx.sh_ptrs + nshift, r, g_ctx.sh_coef,
+                                   Nshift2, n, g_ctx.sh_partial, g_ctx.sh_counters, g_ctx.sh_pp,
+                                   st));
+        double* hpp = g_ctx.host_coef + 5 * nshift;
+        HIPCHK(hipMemcpyAsync(hpp, g_ctx.sh_pp, sizeof(double) * Nshift2,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int ish = 1; ish < Nshift2; ++ish) {
+          pp[ish] = hpp[ish] * snorm;
+          zeta2[ish] = zeta1[ish];
+          zeta1[ish] = znew[ish];
+        }
+      }
+      int ish1 = Nshift2;
+      for (int ish = Nshift2 - 1; ish >= 0; --ish)
+        if (pp[ish] > stop_cond) {
+          ish1 = ish + 1;
+          break;
+        }
+      Nshift2 = ish1;
+      alpha_p = alpha;
+      beta_p = beta;
+      Nconv2 += 1;
+      if (rr * snorm < stop_cond) {
+        converged = true;
+        break;
+      }
+    }
+    double dmax = -1.0;
+    for (int i = 0; i < nshift; ++i) {
+      RC(mult_impl(op, s, xq[i], op->mode, false));
+      HIPCHK(launch_axpy(P64, s, sigma[i], xq[i], n, st));
+      double d2;
+      HIPCHK(launch_sub_norm2(P64, s, s, b, n, g_ctx.red, 0, st));
+      RC(fetch(1, &d2));
+      const double d1 = std::sqrt(d2 * snorm);
+      if (d1 > dmax) dmax = d1;
+    }
+    *nconv_out = Nconv2;
+    *diff_out = dmax;
+    return converged ? DWF_OK : fail(DWF_ERR_NOT_CONVERGED, "AShiftsolver_CG: not converged");
+  };
+  rc = body();
+  release();
+  return rc;
+}
+
+// ---- smearing ---------------------------------------------------------------
+struct dwf_smear_s {
+  int N[4];
+  double rho;
+  int nsmear;
+  size_t V;
+  std::vector<double*> levels;
+};
+
+int dwf_smear_create(const int dims[4], double rho, int nsmear, dwf_smear* out) {
+  CTX();
+  if (!dims || !out || nsmear < 0) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  dwf_smear_s* sm = new dwf_smear_s;
+  sm->V = 1;
+  for (int mu = 0; mu < 4; ++mu) {
+    sm->N[mu] = dims[mu];
+    sm->V *= dims[mu];
+  }
+  sm->rho = rho;
+  sm->nsmear = nsmear;
+  sm->levels.assign(nsmear + 1, nullptr);
+  for (auto& l : sm->levels)
+    if (hipMalloc(&l, sm->V * 4 * 18 * sizeof(double)) != hipSuccess) {
+      dwf_smear_destroy(sm);
+      return fail(DWF_ERR_HIP, "device allocation failed");
+    }
+  *out = sm;
+  return DWF_OK;
+}
+int dwf_smear_destroy(dwf_smear sm) {
+  if (!sm) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  for (auto l : sm->levels)
+    if (l) (void)hipFree(l);
+  delete sm;
+  return DWF_OK;
+}
+static int smear_levels(dwf_smear sm) {
+  for (int i = 0; i < sm->nsmear; ++i)
+    HIPCHK(launch_stout(sm->levels[i + 1], sm->levels[i], sm->N[0], sm->N[1], sm->N[2], sm->N[3],
+                        sm->rho, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_smear_set_config_device(dwf_smear sm, const double* U) {
+  CTX();
+  if (!sm || !U) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(sm->levels[0], U, sm->V * 4 * 18 * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_ctx.stream));
+  return smear_levels(sm);
+}
+int dwf_smear_set_config(dwf_smear sm, const double* U_host) {
+  CTX();
+  if (!sm || !U_host) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(sm->levels[0], U_host, sm->V * 4 * 18 * sizeof(double),
+                        hipMemcpyHostToDevice, g_ctx.stream));
+  RC(smear_levels(sm));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_smear_get_config(dwf_smear sm, int level, double* U_host) {
+  CTX();
+  if (!sm || !U_host || level < 0 || level > sm->nsmear)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  HIPCHK(hipMemcpyAsync(U_host, sm->levels[level], sm->V * 4 * 18 * sizeof(double),
+                        hipMemcpyDeviceToHost, g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+const double* dwf_smear_config_device(dwf_smear sm, int level) {
+  if (!sm || level < 0 || level > sm->nsmear) return nullptr;
+  return sm->levels[level];
+}
+int dwf_fopr_set_config_smeared(dwf_fopr op, dwf_smear sm, const double* U_host) {
+  CTX();
+  if (!op || !sm) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  for (int mu = 0; mu < 4; ++mu)
+    if (sm->N[mu] != op->N[mu]) return fail(DWF_ERR_INVALID_ARG, "lattice sizes differ");
+  if (U_host) RC(dwf_smear_set_config(sm, U_host));
+  RC(dwf_fopr_set_config_device(op, sm->levels[sm->nsmear]));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_plaquette(const int dims[4], const double* U_host, double* result) {
+  CTX();
+  if (!dims || !U_host || !result) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  double* d = nullptr;
+  HIPCHK(hipMalloc(&d, V * 4 * 18 * sizeof(double)));
+  int rc = DWF_OK;
+  if (hipMemcpyAsync(d, U_host, V * 4 * 18 * sizeof(double), hipMemcpyHostToDevice,
+                     g_ctx.stream) != hipSuccess ||
+      launch_plaquette(d, dims[0], dims[1], dims[2], dims[3], g_ctx.red, 0, g_ctx.stream) !=
+          hipSuccess)
+    rc = fail(DWF_ERR_HIP, "plaquette kernel failed");
+  double sum = 0;
+  if (rc == DWF_OK) rc = fetch(1, &sum);
+  (void)hipFree(d);
+  if (rc == DWF_OK) *result = sum / (3.0 * 6.0 * (double)V);
+  return rc;
+}
+
+}  // extern "C"

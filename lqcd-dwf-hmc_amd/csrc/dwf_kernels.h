@@ -1,0 +1,99 @@
+// dwf_kernels.h -- host-side launchers of the libdwfhip kernels.
+//
+// Device field layout of a 5D even-odd spinor (one parity):
+//   element (site, k, s) at ((site * 24) + k) * Ls + s
+//   site in [0, V/2) (even-odd half-site order of the reference), k = ri + 2*(c + 3*d)
+//   (Dirac basis), s in [0, Ls).  Ls is innermost so that the Ls threads of one
+//   4D site read contiguous memory and share that site's links.
+// Device links (per output parity p, hop direction dir = 2*mu + fb):
+//   element (p, dir, ab, site) at ((p * 8 + dir) * 18 + ab) * Vh + site
+//   fb = 0: U_mu(x) times the boundary factor when x + mu wraps
+//   fb = 1: U_mu(x - mu)^dagger times the boundary factor when x - mu wraps
+// Neighbour table: nbr[(p * 8 + dir) * Vh + site] = half-site of x +/- mu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace dwf {
+
+enum Prec { P64 = 0, P32 = 1, P16 = 2, PBF16 = 3 };
+enum KOp { K_SITE = 0, K_HOP = 1, K_HOPDAG = 2 };
+
+size_t prec_bytes(int prec);
+
+// Epilogue of the fused kernel:  out = xcoef * X + E(core),  out2 = F(out)
+// core = hop(in) (K_HOP), gamma5 hop gamma5 (in) (K_HOPDAG) or in (K_SITE).
+// E, F: per-chirality Ls x Ls real matrices in the compute type, laid out
+// [chi][s_out][s_in] with chi = 0 for P+ and 1 for P-, each already scaled by
+// 1/2 (the chiral recombination factor).  E == nullptr means out = scale*core.
+struct Epi {
+  const void* E = nullptr;
+  double scale = 1.0;
+  const void* X = nullptr;
+  double xcoef = 0.0;
+  void* out2 = nullptr;
+  const void* F = nullptr;
+};
+
+hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
+                      const int* nbr, int parity, int Vh, int Ls, const Epi& e,
+                      hipStream_t st);
+
+hipError_t launch_to_device_layout(int prec, void* out, const double* in, int Vh, int Ls,
+                                   hipStream_t st);
+hipError_t launch_to_corelib_layout(int prec, double* out, const void* in, int Vh, int Ls,
+                                    hipStream_t st);
+// U: lexical Field_G (4, V, 18) double on the device.
+hipError_t launch_pack_links(int prec, void* links, const double* U, int Nx, int Ny, int Nz,
+                             int Nt, int bcx, int bcy, int bcz, int bct, hipStream_t st);
+
+// BLAS-1 on n elements of the storage type (arithmetic in the compute type,
+// reductions in double).
+hipError_t launch_set_zero(int prec, void* x, size_t n, hipStream_t st);
+hipError_t launch_copy(int prec, void* y, const void* x, size_t n, hipStream_t st);
+hipError_t launch_axpy(int prec, void* y, double a, const void* x, size_t n, hipStream_t st);
+hipError_t launch_aypx(int prec, double a, void* y, const void* x, size_t n, hipStream_t st);
+hipError_t launch_scal(int prec, void* x, double a, size_t n, hipStream_t st);
+hipError_t launch_convert(int prec_out, void* y, int prec_in, const void* x, size_t n,
+                          hipStream_t st);
+
+// Reductions: write to result[0..] (device), using the scratch partial/counter.
+struct RedScratch {
+  double* partial;   // kRedBlocks * 4 doubles
+  unsigned* counter; // 1
+  double* result;    // 8
+};
+constexpr int kRedBlocks = 1024;
+hipError_t launch_norm2(int prec, const void* x, size_t n, const RedScratch& r, int slot,
+                        hipStream_t st);
+hipError_t launch_dot(int prec, const void* x, const void* y, size_t n, const RedScratch& r,
+                      int slot, hipStream_t st);
+// complex <x, y> (result[slot] = Re, result[slot+1] = Im); n = Vh * 24 * Ls
+hipError_t launch_dotc_ls(int prec, const void* x, const void* y, size_t n, int Ls,
+                          const RedScratch& r, int slot, hipStream_t st);
+// CG step:  x += a p;  r -= a s;  result[slot] = |r|^2
+hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s, double a,
+                        size_t n, const RedScratch& red, int slot, hipStream_t st);
+// y += a x; result[slot] = Re<y, x>   (used as s += sigma0 p; pap = <s, p>)
+hipError_t launch_axpy_dot(int prec, void* y, double a, const void* x, size_t n,
+                           const RedScratch& red, int slot, hipStream_t st);
+// r = (A x) - b, result[slot] = |r|^2   (r may alias Ax)
+hipError_t launch_sub_norm2(int prec, void* r, const void* ax, const void* b, size_t n,
+                            const RedScratch& red, int slot, hipStream_t st);
+
+// Multishift update of active shifts j = 1..m-1 (arrays of device pointers on
+// the device; coefficients on the device, 3 per shift: -beta_s, alpha_s, zeta):
+//   x_j += c0 p_j;  p_j = c1 p_j + c2 r;  pp[j] = |p_j|^2
+hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
+                               const double* coef, int m, size_t n, double* partial,
+                               unsigned* counters, double* pp, hipStream_t st);
+
+// Gauge: one stout smearing level, lexical Field_G (4, V, 18) double device.
+hipError_t launch_stout(double* Uout, const double* Uin, int Nx, int Ny, int Nz, int Nt,
+                        double rho, hipStream_t st);
+// plaquette sum (un-normalised) into result[slot]
+hipError_t launch_plaquette(const double* U, int Nx, int Ny, int Nz, int Nt,
+                            const RedScratch& red, int slot, hipStream_t st);
+
+}  // namespace dwf

@@ -1,0 +1,348 @@
+"""dwf_amd -- Python mirror of the reference's AFopr / ASolver / AShiftsolver /
+Director_alt_Smear interfaces, over the C ABI of libdwfhip (HIP, gfx950).
+
+Class and method names follow the reference (src/lib/Fopr/afopr.h:47-277,
+src/lib_alt/Solver/asolver.h:22-79, src/lib/Solver/ashiftsolver.h:26-54,
+src/lib_alt/Smear/director_alt_Smear.h); fields are device resident (AFIELD),
+host data crosses the boundary only through convert/reverse in the corelib
+layouts described in dwf_amd.lattice.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DwfError, F64, F32, F16, BF16, GUESS_RHS, GUESS_ZERO, GUESS_GIVEN
+
+_PREC = {"double": F64, "float": F32, "half": F16, "bfloat16": BF16,
+         F64: F64, F32: F32, F16: F16, BF16: BF16}
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def stream():
+    """The library's HIP stream (hipStream_t as an int)."""
+    return _lib.load().dwf_stream()
+
+
+def synchronize():
+    _lib.check("dwf_synchronize", _lib.load().dwf_synchronize())
+
+
+class Field:
+    """A device-resident 5D even-odd field of one operator (AFIELD)."""
+
+    def __init__(self, fopr):
+        L = _lib.load()
+        self.fopr = fopr
+        p = C.c_void_p()
+        _lib.check("dwf_field_alloc", L.dwf_field_alloc(fopr._h, C.byref(p)))
+        self._p = p.value
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def free(self):
+        if getattr(self, "_p", None):
+            _lib.load().dwf_field_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # BLAS-1 (AField methods)
+    def set_zero(self):
+        _lib.check("dwf_field_set_zero", _lib.load().dwf_field_set_zero(self.fopr._h, self._p))
+
+    def copy_from(self, other):
+        _lib.check("dwf_field_copy", _lib.load().dwf_field_copy(self.fopr._h, self._p, other._p))
+
+    def axpy(self, a, x):
+        _lib.check("dwf_field_axpy",
+                   _lib.load().dwf_field_axpy(self.fopr._h, self._p, float(a), x._p))
+
+    def aypx(self, a, x):
+        _lib.check("dwf_field_aypx",
+                   _lib.load().dwf_field_aypx(self.fopr._h, float(a), self._p, x._p))
+
+    def scal(self, a):
+        _lib.check("dwf_field_scal", _lib.load().dwf_field_scal(self.fopr._h, self._p, float(a)))
+
+    def norm2(self):
+        r = np.zeros(1)
+        _lib.check("dwf_field_norm2", _lib.load().dwf_field_norm2(self.fopr._h, self._p, _ptr(r)))
+        return float(r[0])
+
+    def dot(self, other):
+        r = np.zeros(1)
+        _lib.check("dwf_field_dot",
+                   _lib.load().dwf_field_dot(self.fopr._h, self._p, other._p, _ptr(r)))
+        return float(r[0])
+
+    def dotc(self, other):
+        re, im = np.zeros(1), np.zeros(1)
+        _lib.check("dwf_field_dotc", _lib.load().dwf_field_dotc(
+            self.fopr._h, self._p, other._p, _ptr(re), _ptr(im)))
+        return complex(re[0], im[0])
+
+    def convert_precision_from(self, other):
+        _lib.check("dwf_field_convert_precision", _lib.load().dwf_field_convert_precision(
+            self.fopr._h, self._p, other.fopr._h, other._p))
+
+
+class AFopr_Domainwall_5din_eo:
+    """Even-odd Moebius domain-wall operator on the GPU.
+
+    Mirrors AFopr<AFIELD> for fermion_type "Domainwall_5din_eo"; parameters
+    carry the YAML names of the reference (quark_mass, domain_wall_height,
+    extent_of_5th_dimension, coefficient_b, coefficient_c, boundary_condition).
+    """
+
+    def __init__(self, lattice_size, extent_of_5th_dimension, quark_mass,
+                 domain_wall_height, coefficient_b=1.0, coefficient_c=0.0,
+                 boundary_condition=(1, 1, 1, -1), precision="double", parameter_alpha=1.0):
+        L = _lib.load()
+        Ls = int(extent_of_5th_dimension)
+        prm = _lib.FoprParams()
+        for i in range(4):
+            prm.lattice_size[i] = int(lattice_size[i])
+            prm.boundary_condition[i] = int(boundary_condition[i])
+        prm.extent_of_5th_dimension = Ls
+        prm.quark_mass = float(quark_mass)
+        prm.domain_wall_height = float(domain_wall_height)
+        self._b = None
+        self._c = None
+        if np.isscalar(coefficient_b):
+            prm.b_uniform = float(coefficient_b)
+        else:
+            self._b = np.ascontiguousarray(coefficient_b, np.float64)
+            prm.coefficient_b = _ptr(self._b)
+        if np.isscalar(coefficient_c):
+            prm.c_uniform = float(coefficient_c)
+        else:
+            self._c = np.ascontiguousarray(coefficient_c, np.float64)
+            prm.coefficient_c = _ptr(self._c)
+        prm.parameter_alpha = float(parameter_alpha)
+        prm.precision = _PREC[precision]
+        h = C.c_void_p()
+        _lib.check("dwf_fopr_create", L.dwf_fopr_create(C.byref(prm), C.byref(h)))
+        self._h = h.value
+        self.lattice_size = tuple(int(v) for v in lattice_size)
+        self.Ls = Ls
+        self.Vh = L.dwf_fopr_field_nvol(self._h)
+        self.precision = prm.precision
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.load().dwf_fopr_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def new_field(self):
+        return Field(self)
+
+    def set_config(self, U):
+        U = np.ascontiguousarray(U, np.float64)
+        _lib.check("dwf_fopr_set_config", _lib.load().dwf_fopr_set_config(self._h, _ptr(U)))
+
+    def set_config_smeared(self, smear, U=None):
+        """AFopr_Smeared_alt::set_config: smear U, give the top level to the kernel."""
+        Up = None if U is None else _ptr(np.ascontiguousarray(U, np.float64))
+        _lib.check("dwf_fopr_set_config_smeared",
+                   _lib.load().dwf_fopr_set_config_smeared(self._h, smear._h, Up))
+
+    def set_mode(self, mode):
+        _lib.check("dwf_fopr_set_mode", _lib.load().dwf_fopr_set_mode(self._h, mode.encode()))
+
+    def get_mode(self):
+        return _lib.load().dwf_fopr_get_mode(self._h).decode()
+
+    def mult(self, v, w, mode=None):
+        L = _lib.load()
+        if mode is None:
+            _lib.check("dwf_fopr_mult", L.dwf_fopr_mult(self._h, v.ptr, w.ptr))
+        else:
+            _lib.check("dwf_fopr_mult_mode",
+                       L.dwf_fopr_mult_mode(self._h, v.ptr, w.ptr, mode.encode()))
+
+    def mult_dag(self, v, w, mode=None):
+        L = _lib.load()
+        if mode is None:
+            _lib.check("dwf_fopr_mult_dag", L.dwf_fopr_mult_dag(self._h, v.ptr, w.ptr))
+        else:
+            _lib.check("dwf_fopr_mult_dag_mode",
+                       L.dwf_fopr_mult_dag_mode(self._h, v.ptr, w.ptr, mode.encode()))
+
+    def mult_gm5(self, v, w):
+        _lib.check("dwf_fopr_mult_gm5", _lib.load().dwf_fopr_mult_gm5(self._h, v.ptr, w.ptr))
+
+    def mult_gm5R(self, v, w):
+        _lib.check("dwf_fopr_mult_gm5R", _lib.load().dwf_fopr_mult_gm5R(self._h, v.ptr, w.ptr))
+
+    def field_nin(self):
+        return _lib.load().dwf_fopr_field_nin(self._h)
+
+    def field_nvol(self):
+        return _lib.load().dwf_fopr_field_nvol(self._h)
+
+    def field_nex(self):
+        return _lib.load().dwf_fopr_field_nex(self._h)
+
+    def field_bytes(self):
+        return _lib.load().dwf_fopr_field_bytes(self._h)
+
+    def flop_count(self, mode):
+        return _lib.load().dwf_fopr_flop_count(self._h, mode.encode())
+
+    def convert(self, v, w):
+        """corelib Field(24, V/2, Ls) host array w -> device field v."""
+        w = np.ascontiguousarray(w, np.float64)
+        assert w.size == 24 * self.Vh * self.Ls
+        _lib.check("dwf_fopr_convert", _lib.load().dwf_fopr_convert(self._h, v.ptr, _ptr(w)))
+
+    def reverse(self, w):
+        """device field w -> corelib Field(24, V/2, Ls) host array (Ls, V/2, 24)."""
+        out = np.empty((self.Ls, self.Vh, 24))
+        _lib.check("dwf_fopr_reverse", _lib.load().dwf_fopr_reverse(self._h, _ptr(out), w.ptr))
+        return out
+
+    # convenience: host in, host out
+    def apply(self, mode, w_host, dagger=False):
+        w = self.new_field()
+        v = self.new_field()
+        self.convert(w, w_host)
+        saved = self.get_mode()
+        if mode in ("Deo", "Doe", "Dee", "Doo", "Dee_inv", "Doo_inv"):
+            (self.mult_dag if dagger else self.mult)(v, w, mode)
+        else:
+            self.set_mode(mode)
+            (self.mult_dag if dagger else self.mult)(v, w)
+            self.set_mode(saved)
+        return self.reverse(v)
+
+
+class ASolver_CG:
+    """ASolver_CG on fopr.mult in fopr's current mode (asolver_CG-tmpl.h)."""
+
+    def __init__(self, fopr, maximum_number_of_iteration, convergence_criterion_squared,
+                 initial_guess_mode="RHS"):
+        g = {"RHS": GUESS_RHS, "ZERO": GUESS_ZERO, "GIVEN": GUESS_GIVEN}[initial_guess_mode]
+        h = C.c_void_p()
+        _lib.check("dwf_solver_create_cg", _lib.load().dwf_solver_create_cg(
+            fopr._h, int(maximum_number_of_iteration), float(convergence_criterion_squared), g,
+            C.byref(h)))
+        self._h = h.value
+        self.fopr = fopr
+
+    def solve(self, x, b, raise_on_nonconvergence=False):
+        """Returns (nconv, diff); nconv = -1 when not converged."""
+        n = C.c_int(0)
+        d = np.zeros(1)
+        rc = _lib.load().dwf_solver_solve(self._h, x.ptr, b.ptr, C.byref(n), _ptr(d))
+        _lib.check("dwf_solver_solve", rc,
+                   allow=() if raise_on_nonconvergence else (_lib.ERR_NOT_CONVERGED,))
+        return n.value, float(d[0])
+
+    def inner_iterations(self):
+        return _lib.load().dwf_solver_inner_iterations(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.load().dwf_solver_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class ASolver_Richardson(ASolver_CG):
+    """Richardson on fopr_d, preconditioned by APrecond_Mixedprec with an inner
+    ASolver_CG on fopr_f from a zero guess (fprop_alt_Standard_eo_Richardson)."""
+
+    def __init__(self, fopr_d, fopr_f, maximum_number_of_iteration,
+                 convergence_criterion_squared, inner_iteration=100000,
+                 inner_convergence_criterion_squared=1.0e-8):
+        h = C.c_void_p()
+        _lib.check("dwf_solver_create_richardson", _lib.load().dwf_solver_create_richardson(
+            fopr_d._h, fopr_f._h, int(maximum_number_of_iteration),
+            float(convergence_criterion_squared), int(inner_iteration),
+            float(inner_convergence_criterion_squared), C.byref(h)))
+        self._h = h.value
+        self.fopr = fopr_d
+        self.fopr_f = fopr_f
+
+
+class AShiftsolver_CG:
+    """Multishift CG (ashiftsolver_CG-tmpl.h) on fopr.mult + sigma_i."""
+
+    def __init__(self, fopr, maximum_number_of_iteration, convergence_criterion_squared):
+        self.fopr = fopr
+        self.niter = int(maximum_number_of_iteration)
+        self.stop = float(convergence_criterion_squared)
+
+    def solve(self, xs, sigma, b, raise_on_nonconvergence=True):
+        sig = np.ascontiguousarray(sigma, np.float64)
+        arr = (C.c_void_p * len(xs))(*[x.ptr for x in xs])
+        n = C.c_int(0)
+        d = np.zeros(1)
+        rc = _lib.load().dwf_shiftsolver_cg_solve(self.fopr._h, self.niter, self.stop, len(xs),
+                                                  _ptr(sig), arr, b.ptr, C.byref(n), _ptr(d))
+        _lib.check("dwf_shiftsolver_cg_solve", rc,
+                   allow=() if raise_on_nonconvergence else (_lib.ERR_NOT_CONVERGED,))
+        return n.value, float(d[0])
+
+
+class Director_alt_Smear:
+    """Stout smearing levels (APE staples + stout projection) on the GPU."""
+
+    def __init__(self, lattice_size, rho_uniform, number_of_smearing):
+        h = C.c_void_p()
+        dims = (C.c_int * 4)(*[int(v) for v in lattice_size])
+        _lib.check("dwf_smear_create", _lib.load().dwf_smear_create(
+            dims, float(rho_uniform), int(number_of_smearing), C.byref(h)))
+        self._h = h.value
+        self.lattice_size = tuple(int(v) for v in lattice_size)
+        self.Nsmear = int(number_of_smearing)
+
+    def set_config(self, U):
+        U = np.ascontiguousarray(U, np.float64)
+        _lib.check("dwf_smear_set_config", _lib.load().dwf_smear_set_config(self._h, _ptr(U)))
+
+    def get_config(self, level):
+        V = int(np.prod(self.lattice_size))
+        out = np.empty((4, V, 18))
+        _lib.check("dwf_smear_get_config",
+                   _lib.load().dwf_smear_get_config(self._h, int(level), _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.load().dwf_smear_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def plaquette(lattice_size, U):
+    U = np.ascontiguousarray(U, np.float64)
+    r = np.zeros(1)
+    dims = (C.c_int * 4)(*[int(v) for v in lattice_size])
+    _lib.check("dwf_plaquette", _lib.load().dwf_plaquette(dims, _ptr(U), _ptr(r)))
+    return float(r[0])
