@@ -1,0 +1,289 @@
+#!/usr/bin/env python3
+"""Benchmark of the domain-wall hot path on one MI355X.
+
+Workload: the mixed-precision solve on a 16^3 x 32 lattice, Ls = 16,
+mq = 0.01, M0 = 1.6, Shamir (b = 1, c = 0), antiperiodic in t, links
+U = exp(i 0.3 H) with H Gaussian traceless Hermitian, stout-smeared twice with
+rho = 0.1 on the GPU; Gaussian complex source on the even sites; target
+|r|/|b| < 1e-10 on the FP64 true residual of DdagD x = b.
+
+One timed step = one solve by FP64 Richardson with an FP32 inner CG (the
+reference's Fprop_alt_Standard_eo_Richardson scheme).  Also measured, outside
+the timed steps: the plain FP64 CG solve of the same system, the same
+Richardson with a bfloat16-storage inner operator, and the FP64 even-odd
+Dslash D_eo (one fused hop kernel) timed with HIP events on the library
+stream, reported as algorithmic HBM GB/s against the 8 TB/s MI355X peak.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lqcd-dwf-hmc_amd"))
+
+METRIC = "time-to-|r|<1e-10, mixed+learned vs FP64 CG; Dslash HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+DIMS = (16, 16, 16, 32)
+LS = 16
+MQ, M0, B, CC = 0.01, 1.6, 1.0, 0.0
+EPS, RHO, NSMEAR = 0.3, 0.1, 2
+TOL2 = 1e-20          # |r|^2 / |b|^2  <->  |r|/|b| < 1e-10
+INNER_TOL2 = 1e-8     # fprop_alt_Standard_eo_Richardson-tmpl.h:60 (inner criterion)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on a given stream through the HIP runtime."""
+
+    def __init__(self):
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.L = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        else:
+            raise RuntimeError("libamdhip64.so not found")
+        self.L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.L.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        if self.L.hipEventCreate(C.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return e
+
+    def record(self, e, stream):
+        if self.L.hipEventRecord(e, C.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, a, b):
+        self.L.hipEventSynchronize(b)
+        ms = C.c_float()
+        if self.L.hipEventElapsedTime(C.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value
+
+
+def deo_bytes(Vh, Ls, elem):
+    """Algorithmic bytes of one D_eo: read the input spinor of the other parity,
+    all links (8 per output site), write the output spinor."""
+    spinor = Vh * Ls * 24 * elem
+    links = Vh * 8 * 18 * elem
+    return 2 * spinor + links
+
+
+def cpu_baseline(U_smeared, x, seconds):
+    """Oracle D_eo (FP64, corelib-style restatement) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import OracleDWF, num_threads
+    orc = OracleDWF(DIMS, LS, MQ, M0, B, CC)
+    orc.set_config(U_smeared)
+    orc.mult("Deo", x)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.mult("Deo", x)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = (time.perf_counter() - t0) / n
+    Vh = DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3] // 2
+    return {
+        "value": round(deo_bytes(Vh, LS, 8) / dt / 1e9, 3),
+        "unit": "GB/s",
+        "cores": num_threads(),
+        "kind": "port",
+        "sample": f"{n} calls of the oracle's FP64 D_eo on 16^3x32, Ls=16 "
+                  f"({dt * 1e3:.1f} ms/call); same algorithmic bytes as the GPU figure",
+    }
+
+
+def load_traffic(key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        v = data.get(key, {}).get("bytes_per_launch")
+        return None if v is None else float(v)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dslash-reps", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-cg", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the lattice would shard by 4D domain decomposition; see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import dwf_amd
+    from dwf_amd import lattice
+
+    t_setup = time.perf_counter()
+    U0 = lattice.random_gauge(DIMS, EPS, 2024)
+    b_host = lattice.random_spinor5(DIMS, LS, 7)
+    sm = dwf_amd.Director_alt_Smear(DIMS, RHO, NSMEAR)
+    f64 = dwf_amd.AFopr_Domainwall_5din_eo(DIMS, LS, MQ, M0, B, CC, (1, 1, 1, -1), "double")
+    f32 = dwf_amd.AFopr_Domainwall_5din_eo(DIMS, LS, MQ, M0, B, CC, (1, 1, 1, -1), "float")
+    fbf = dwf_amd.AFopr_Domainwall_5din_eo(DIMS, LS, MQ, M0, B, CC, (1, 1, 1, -1), "bfloat16")
+    f64.set_config_smeared(sm, U0)
+    f32.set_config_smeared(sm, None)
+    fbf.set_config_smeared(sm, None)
+    for f in (f64, f32, fbf):
+        f.set_mode("DdagD")
+    b = f64.new_field()
+    x = f64.new_field()
+    f64.convert(b, b_host)
+    log(f"setup {time.perf_counter() - t_setup:.1f} s; lattice {DIMS} Ls={LS}")
+
+    # ---- Dslash: FP64 D_eo, HIP events on the library stream ---------------
+    ev = HipEvents()
+    st = dwf_amd.stream()
+    v = f64.new_field()
+    for _ in range(5):
+        f64.mult(v, b, "Deo")
+    dwf_amd.synchronize()
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0, st)
+    for _ in range(a.dslash_reps):
+        f64.mult(v, b, "Deo")
+    ev.record(e1, st)
+    deo_ms = ev.elapsed_ms(e0, e1) / a.dslash_reps
+    Vh = f64.Vh
+    deo_b = deo_bytes(Vh, LS, 8)
+    deo_gbs = deo_b / (deo_ms * 1e-3) / 1e9
+    log(f"D_eo FP64: {deo_ms * 1e3:.1f} us/launch, {deo_gbs:.0f} GB/s algorithmic "
+        f"({deo_b / 1e6:.1f} MB/launch), {deo_gbs / HBM_PEAK_GBS * 100:.1f}% of 8 TB/s")
+    # FP32 and DdagD figures for the record
+    v32, b32 = f32.new_field(), f32.new_field()
+    b32.convert_precision_from(b)
+    ev.record(e0, st)
+    for _ in range(a.dslash_reps):
+        f32.mult(v32, b32, "Deo")
+    ev.record(e1, st)
+    deo32_ms = ev.elapsed_ms(e0, e1) / a.dslash_reps
+    ev.record(e0, st)
+    for _ in range(20):
+        f64.mult(v, b)
+    ev.record(e1, st)
+    ddd_ms = ev.elapsed_ms(e0, e1) / 20
+    log(f"D_eo FP32: {deo32_ms * 1e3:.1f} us; DdagD FP64: {ddd_ms * 1e3:.1f} us")
+
+    # ---- timed steps: mixed-precision solves --------------------------------
+    rich = dwf_amd.ASolver_Richardson(f64, f32, 100, TOL2, 100000, INNER_TOL2)
+    for i in range(a.warmup):
+        n, d = rich.solve(x, b)
+        log(f"warmup mixed solve: nconv={n} |r|/|b|={d:.3e} inner={rich.inner_iterations()}")
+    dwf_amd.synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for i in range(a.steps):
+        n, d = rich.solve(x, b)
+        results.append((n, d, rich.inner_iterations()))
+    dwf_amd.synchronize()
+    t_mixed = (time.perf_counter() - t0) / a.steps
+    for n, d, inner in results:
+        if not (n >= 0 and d < 1e-10):
+            raise RuntimeError(f"mixed solve missed the target: nconv={n} |r|/|b|={d}")
+    log(f"mixed FP64/FP32 solve: {t_mixed:.3f} s, outer={results[-1][0]}, "
+        f"inner CG iterations={results[-1][2]}, |r|/|b|={results[-1][1]:.3e}")
+
+    # ---- comparisons (not in the timed steps) --------------------------------
+    t_fp64 = fp64_iters = fp64_diff = None
+    if not a.no_fp64_cg:
+        cg = dwf_amd.ASolver_CG(f64, 100000, TOL2, "ZERO")
+        dwf_amd.synchronize()
+        t0 = time.perf_counter()
+        fp64_iters, fp64_diff = cg.solve(x, b)
+        dwf_amd.synchronize()
+        t_fp64 = time.perf_counter() - t0
+        log(f"plain FP64 CG: {t_fp64:.3f} s, {fp64_iters} iterations, |r|/|b|={fp64_diff:.3e}")
+    richbf = dwf_amd.ASolver_Richardson(f64, fbf, 100, TOL2, 100000, INNER_TOL2)
+    n, d = richbf.solve(x, b)
+    dwf_amd.synchronize()
+    t0 = time.perf_counter()
+    nbf, dbf = richbf.solve(x, b)
+    dwf_amd.synchronize()
+    t_bf = time.perf_counter() - t0
+    log(f"mixed FP64/BF16-storage solve: {t_bf:.3f} s, outer={nbf}, "
+        f"inner={richbf.inner_iterations()}, |r|/|b|={dbf:.3e}")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        Us = sm.get_config(NSMEAR)
+        cpu = cpu_baseline(Us, b_host, a.cpu_seconds)
+        log(f"CPU baseline: {cpu}")
+
+    traffic = load_traffic("k_dwf_hop_f64_16x16x16x32_Ls16")
+    out = {
+        "metric": METRIC,
+        "value": round(deo_gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "mixed-precision solve DdagD x = b to |r|/|b|<1e-10, 16^3x32, Ls=16, "
+                        "mq=0.01, M0=1.6, Shamir, exp(0.3 iH) links + 2 stout(0.1) steps",
+            "lattice": list(DIMS), "Ls": LS, "step": "one FP64-Richardson/FP32-CG solve",
+            "value": "FP64 D_eo algorithmic HBM GB/s",
+        },
+        "time_to_solution_s": {
+            "fp64_cg": None if t_fp64 is None else round(t_fp64, 4),
+            "fp64_cg_iterations": fp64_iters,
+            "mixed_fp64_fp32": round(t_mixed, 4),
+            "mixed_fp64_bf16_storage": round(t_bf, 4),
+            "mixed_learned": None,
+            "speedup_mixed_fp32_vs_fp64": None if t_fp64 is None else round(t_fp64 / t_mixed, 3),
+            "speedup_mixed_bf16_vs_fp64": None if t_fp64 is None else round(t_fp64 / t_bf, 3),
+        },
+        "kernels_us": {"deo_f64": round(deo_ms * 1e3, 2), "deo_f32": round(deo32_ms * 1e3, 2),
+                       "ddagd_f64": round(ddd_ms * 1e3, 2)},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(deo_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(deo_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_dwf hop FP64 (D_eo, output parity even)",
+            "bytes_per_launch": deo_b,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,7 +58,25 @@ def build(jobs=3, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_host_example(verbose)
     return OUT
+
+
+EXAMPLE = os.path.join(HERE, "host", "example_solve")
+
+
+def build_host_example(verbose=False):
+    """The C++ host-side example driver (g++, links libdwfhip.so)."""
+    libdir = os.path.dirname(OUT)
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "host"), os.path.join(HERE, "host", "example_solve.cpp"),
+           "-L", libdir, "-ldwfhip", f"-Wl,-rpath,{libdir}", "-o", EXAMPLE + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host example build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(EXAMPLE + ".tmp", EXAMPLE)
 
 
 def main():

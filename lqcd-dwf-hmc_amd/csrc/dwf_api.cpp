@@ -231,6 +231,8 @@ struct dwf_fopr_s {
   int bc[4];
   std::string mode = "D";
   bool mp_scalar = false;
+  bool deo_is_scalar = false;   // the Deo/Doe epilogue is a multiple of 1
+  double deo_scale = 0.0;
   size_t nelem = 0;         // reals per field
   void* links = nullptr;
   int* nbr = nullptr;
@@ -315,6 +317,22 @@ static int build_mats(dwf_fopr_s* op) {
     ms[M_DEO] = cm_scale(I, k);
   }
   ms[M_DIT] = DiT;
+  // when the Deo epilogue is s * 1 (Shamir-type, uniform b), the hop kernel
+  // applies the scalar instead of the Ls x Ls matrix
+  {
+    const ChiMat& E = ms[M_DEO];
+    const long double d0 = E.at(0, 0, 0);
+    bool scalar = true;
+    for (int c = 0; c < 2 && scalar; ++c)
+      for (int i = 0; i < L && scalar; ++i)
+        for (int j = 0; j < L; ++j)
+          if (E.at(c, i, j) != (i == j ? d0 : 0.0L)) {
+            scalar = false;
+            break;
+          }
+    op->deo_is_scalar = scalar;
+    op->deo_scale = (double)d0;
+  }
   ms[M_G1] = cm_scale(cm_mul(DiT, MpT), k);
   ms[M_G2] = cm_scale(MpT, -k);
   ms[M_DEODAG] = cm_scale(MpT, k);
@@ -465,6 +483,13 @@ static int op_Deo(dwf_fopr_s* op, void* v, const void* w, int parity) {
     RC(get_work(op, 0, &t0));
     RC(run_site(op, t0, w, M_PRE));
     src = t0;
+  }
+  if (op->deo_is_scalar) {
+    Epi e;
+    e.scale = op->deo_scale;
+    HIPCHK(launch_dwf(op->prec, K_HOP, v, src, op->links, op->nbr, parity, op->Vh, op->Ls, e,
+                      g_ctx.stream));
+    return DWF_OK;
   }
   RC(run_hop(op, false, parity, v, src, M_DEO));
   return DWF_OK;

@@ -118,12 +118,14 @@ __device__ __forceinline__ void su3_mul(const C* u, const C* h, C* a) {
   }
 }
 
+// One hop direction for thread (site, ls).  The block's links (already in
+// the compute type) and neighbour indices sit in LDS: shL[(dir*18 + e)*spb + lsite],
+// shN[dir*spb + lsite].
 template <int DIR, bool DAG, typename T, typename C>
-__device__ __forceinline__ void hop_dir(C* acc, const T* __restrict__ in,
-                                        const T* __restrict__ L, const int* __restrict__ nbr,
-                                        int site, int ls, int Vh, int Ls) {
+__device__ __forceinline__ void hop_dir(C* acc, const T* __restrict__ in, const C* shL,
+                                        const int* shN, int lsite, int ls, int spb, int Ls) {
   using S = St<T>;
-  const int nb = nbr[DIR * Vh + site];
+  const int nb = shN[DIR * spb + lsite];
   const T* wp = in + (size_t)nb * 24 * Ls + ls;
   C w[24];
   if (!DAG) {
@@ -138,36 +140,73 @@ __device__ __forceinline__ void hop_dir(C* acc, const T* __restrict__ in,
   }
   C h1[6], h2[6];
   project<DIR>(w, h1, h2);
-  const T* up = L + (size_t)DIR * 18 * Vh + site;
+  const C* up = shL + DIR * 18 * spb + lsite;
   C u[18];
 #pragma unroll
-  for (int e = 0; e < 18; ++e) u[e] = S::ld(up + (size_t)e * Vh);
+  for (int e = 0; e < 18; ++e) u[e] = up[e * spb];
   C a1[6], a2[6];
   su3_mul(u, h1, a1);
   su3_mul(u, h2, a2);
   reconstruct<DIR>(acc, a1, a2);
 }
 
-template <typename T, int OP, bool MAT, bool ADDX, bool OUT2>
-__global__ void __launch_bounds__(256)
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap the hardware workgroup id so that every XCD walks a contiguous
+// range of sites: neighbouring sites then share an L2.
+__device__ __forceinline__ int xcd_block(int w, int G) {
+  const int q = G >> 3, r = G & 7;
+  const int x = w & 7, i = w >> 3;
+  return x * q + (x < r ? x : r) + i;
+}
+
+// Occupancy target (waves per SIMD) of the fused kernel per storage type; the
+// compiler caps registers to reach it.
+#ifndef DWF_WAVES_F64
+#define DWF_WAVES_F64 3
+#endif
+#ifndef DWF_WAVES_F32
+#define DWF_WAVES_F32 4
+#endif
+#ifndef DWF_HOP_THREADS
+#define DWF_HOP_THREADS 256
+#endif
+template <typename T> struct HopWaves { static constexpr int value = DWF_WAVES_F32; };
+template <> struct HopWaves<double> { static constexpr int value = DWF_WAVES_F64; };
+
+// LSC > 0: Ls and the block shape (256/LSC sites) are compile-time constants,
+// so every load and LDS access uses an immediate offset; LSC = 0: runtime Ls.
+template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC>
+__global__ void __launch_bounds__(DWF_HOP_THREADS, HopWaves<T>::value)
 k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links,
-      const int* __restrict__ nbr, int parity, int Vh, int Ls, int spb,
+      const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
       const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
       const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
       const typename St<T>::C* __restrict__ F) {
   using S = St<T>;
   using C = typename S::C;
+  const int Ls = LSC > 0 ? LSC : Ls_rt;
+  const int spb = LSC > 0 ? DWF_HOP_THREADS / LSC : spb_rt;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* sh_u = reinterpret_cast<C*>(smem_raw);
-  C* sh_l = sh_u + spb * 12 * Ls;
-  C* sh_E = sh_l + spb * 12 * Ls;
+  // region A: hop staging (links, neighbours), later the chiral epilogue buffers
+  const int stageC = (OP == K_SITE) ? 0 : (144 * spb + 8 * spb);
+  const int epiC = MAT ? 24 * spb * Ls : 0;
+  const int regionA = stageC > epiC ? stageC : epiC;
+  C* shA = reinterpret_cast<C*>(smem_raw);
+  C* shL = shA;
+  int* shN = reinterpret_cast<int*>(shA + 144 * spb);
+  C* sh_u = shA;
+  C* sh_l = shA + spb * 12 * Ls;
+  C* sh_E = shA + regionA;
   C* sh_F = sh_E + 2 * Ls * Ls;
 
   const int tid = threadIdx.x;
   const int lsite = tid / Ls;
   const int ls = tid - lsite * Ls;
-  const int site = blockIdx.x * spb + lsite;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int site0 = blk * spb;
+  const int site = site0 + lsite;
   const bool valid = (lsite < spb) && (site < Vh);
+  const int nsite = (Vh - site0) < spb ? (Vh - site0) : spb;
 
   if (MAT) {
     const int LL = Ls * Ls;
@@ -178,6 +217,20 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       if (OUT2) sh_F[chi * LL + si * Ls + so] = F[i];
     }
   }
+  if (OP != K_SITE) {
+    // coalesced cooperative load of the block's 8 x 18 link rows and 8 neighbour rows
+    const T* L = links + (size_t)parity * 8 * 18 * Vh + site0;
+    for (int i = tid; i < 144 * spb; i += blockDim.x) {
+      const int row = i / spb, j = i - row * spb;
+      shL[i] = (j < nsite) ? S::ldn(L + (size_t)row * Vh + j) : C(0);
+    }
+    const int* N = nbr + (size_t)parity * 8 * Vh + site0;
+    for (int i = tid; i < 8 * spb; i += blockDim.x) {
+      const int row = i / spb, j = i - row * spb;
+      shN[i] = (j < nsite) ? N[(size_t)row * Vh + j] : 0;
+    }
+    __syncthreads();
+  }
 
   C acc[24];
 #pragma unroll
@@ -187,34 +240,29 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const T* wp = in + (size_t)site * 24 * Ls + ls;
 #pragma unroll
       for (int k = 0; k < 24; ++k) acc[k] = S::ld(wp + k * Ls);
+    } else if (OP == K_HOP) {
+      hop_dir<0, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<1, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<2, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<3, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<4, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<5, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<6, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<7, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
     } else {
-      const bool dag = (OP == K_HOPDAG);
-      const T* L = links + (size_t)parity * 8 * 18 * Vh;
-      const int* nb = nbr + (size_t)parity * 8 * Vh;
-      if (!dag) {
-        hop_dir<0, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<1, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<2, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<3, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<4, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<5, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<6, false>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<7, false>(acc, in, L, nb, site, ls, Vh, Ls);
-      } else {
-        hop_dir<0, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<1, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<2, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<3, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<4, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<5, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<6, true>(acc, in, L, nb, site, ls, Vh, Ls);
-        hop_dir<7, true>(acc, in, L, nb, site, ls, Vh, Ls);
+      hop_dir<0, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<1, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<2, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<3, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<4, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<5, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<6, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      hop_dir<7, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          C t = acc[k];
-          acc[k] = acc[k + 12];
-          acc[k + 12] = t;
-        }
+      for (int k = 0; k < 12; ++k) {
+        C t = acc[k];
+        acc[k] = acc[k + 12];
+        acc[k + 12] = t;
       }
     }
   }
@@ -226,12 +274,13 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       for (int k = 0; k < 24; ++k) {
         C v = scale * acc[k];
         if (ADDX) v += xcoef * S::ld(X + base + k * Ls);
-        S::st(out + base + k * Ls, v);
+        S::stn(out + base + k * Ls, v);
       }
     }
     return;
   }
 
+  if (OP != K_SITE) __syncthreads();  // staging area is reused below
   // chiral components of the core, per (site, k < 12, s): P+ ~ u, P- ~ l
   if (valid) {
     C* u = sh_u + lsite * 12 * Ls + ls;
@@ -270,7 +319,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       for (int k = 0; k < 24; ++k) o[k] += xcoef * S::ld(X + base + k * Ls);
     }
 #pragma unroll
-    for (int k = 0; k < 24; ++k) S::st(out + base + k * Ls, o[k]);
+    for (int k = 0; k < 24; ++k) S::stn(out + base + k * Ls, o[k]);
   }
   if (OUT2) {
     __syncthreads();
@@ -302,11 +351,54 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       }
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
-        S::st(out2 + base + k * Ls, ou[k] + ol[k]);
-        S::st(out2 + base + (k + 12) * Ls, ou[k] - ol[k]);
+        S::stn(out2 + base + k * Ls, ou[k] + ol[k]);
+        S::stn(out2 + base + (k + 12) * Ls, ou[k] - ol[k]);
       }
     }
   }
+}
+
+template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC>
+static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* out,
+                     const void* in, const void* links, const int* nbr, int parity, int Vh,
+                     int Ls, int spb, const Epi& e) {
+  using C = typename St<T>::C;
+  hipLaunchKernelGGL((k_dwf<T, OP, MAT, ADDX, OUT2, LSC>), grid, block, lds, st, (T*)out,
+                     (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
+                     (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F);
+}
+
+// the (op, epilogue) combinations the operator chains use
+template <typename T, int LSC>
+static hipError_t launch_combo(int op, bool mat, bool addx, bool out2, dim3 g, dim3 b,
+                               size_t lds, hipStream_t st, void* out, const void* in,
+                               const void* links, const int* nbr, int parity, int Vh, int Ls,
+                               int spb, const Epi& e) {
+#define LK(OPV, M, A, O2) \
+  launch_k<T, OPV, M, A, O2, LSC>(g, b, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e)
+  if (op == K_SITE) {
+    if (!mat || addx) return hipErrorInvalidValue;
+    if (out2) LK(K_SITE, true, false, true);
+    else LK(K_SITE, true, false, false);
+  } else if (op == K_HOP) {
+    if (!mat) {
+      if (addx || out2) return hipErrorInvalidValue;
+      LK(K_HOP, false, false, false);
+    } else if (!addx) {
+      if (out2) return hipErrorInvalidValue;
+      LK(K_HOP, true, false, false);
+    } else if (out2) {
+      LK(K_HOP, true, true, true);
+    } else {
+      LK(K_HOP, true, true, false);
+    }
+  } else {
+    if (!mat || out2) return hipErrorInvalidValue;
+    if (addx) LK(K_HOPDAG, true, true, false);
+    else LK(K_HOPDAG, true, false, false);
+  }
+#undef LK
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -318,48 +410,32 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const bool addx = e.X != nullptr;
   const bool out2 = e.out2 != nullptr;
   if (out2 && !mat) return hipErrorInvalidValue;
-  int spb = 256 / Ls;
+  int spb = DWF_HOP_THREADS / Ls;
   if (spb < 1) return hipErrorInvalidValue;
-  size_t matb = (mat ? 2 : 0) * (size_t)Ls * Ls * sizeof(C) * (out2 ? 2 : 1);
+  const size_t cs = sizeof(C);
+  const size_t matb = mat ? 2 * (size_t)Ls * Ls * cs * (out2 ? 2 : 1) : 0;
   const size_t lds_cap = 64 * 1024;
-  if (mat) {
-    if (matb >= lds_cap) return hipErrorInvalidValue;
-    int cap = (int)((lds_cap - matb) / (24 * (size_t)Ls * sizeof(C)));
-    if (cap < spb) spb = cap;
-    if (spb < 1) return hipErrorInvalidValue;
-  }
-  const size_t lds = mat ? (24 * (size_t)spb * Ls * sizeof(C) + matb) : 0;
+  auto lds_for = [&](int sp) {
+    const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * sp * cs;
+    const size_t epi = mat ? 24 * (size_t)sp * Ls * cs : 0;
+    return (stage > epi ? stage : epi) + matb;
+  };
+  const bool full = lds_for(spb) <= lds_cap;
+  while (spb > 1 && lds_for(spb) > lds_cap) --spb;
+  if (lds_for(spb) > lds_cap) return hipErrorInvalidValue;
+  const size_t lds = lds_for(spb);
   dim3 block(spb * Ls), grid((Vh + spb - 1) / spb);
-  T* o = (T*)out;
-  const T* i = (const T*)in;
-  const T* L = (const T*)links;
-  const C* E = (const C*)e.E;
-  const C* F = (const C*)e.F;
-  const T* X = (const T*)e.X;
-  T* o2 = (T*)e.out2;
-  const C sc = (C)e.scale, xc = (C)e.xcoef;
-#define DWF_LAUNCH(OPV, M, A, O2)                                                          \
-  hipLaunchKernelGGL((k_dwf<T, OPV, M, A, O2>), grid, block, lds, st, o, i, L, nbr, parity, \
-                     Vh, Ls, spb, E, sc, X, xc, o2, F)
-#define DWF_BY_OP(M, A, O2)                                \
-  do {                                                     \
-    if (op == K_SITE) DWF_LAUNCH(K_SITE, M, A, O2);        \
-    else if (op == K_HOP) DWF_LAUNCH(K_HOP, M, A, O2);     \
-    else DWF_LAUNCH(K_HOPDAG, M, A, O2);                   \
-  } while (0)
-  if (!mat) {
-    if (addx) DWF_BY_OP(false, true, false);
-    else DWF_BY_OP(false, false, false);
-  } else if (!out2) {
-    if (addx) DWF_BY_OP(true, true, false);
-    else DWF_BY_OP(true, false, false);
-  } else {
-    if (addx) DWF_BY_OP(true, true, true);
-    else DWF_BY_OP(true, false, true);
+  if (full) {
+    switch (Ls) {
+      case 4: return launch_combo<T, 4>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
+      case 8: return launch_combo<T, 8>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
+      case 12: return launch_combo<T, 12>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
+      case 16: return launch_combo<T, 16>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
+      default: break;
+    }
   }
-#undef DWF_BY_OP
-#undef DWF_LAUNCH
-  return hipGetLastError();
+  return launch_combo<T, 0>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr,
+                            parity, Vh, Ls, spb, e);
 }
 
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
@@ -507,43 +583,145 @@ hipError_t launch_pack_links(int prec, void* links, const double* U, int Nx, int
 }
 
 // ---------------------------------------------------------------------------
-// BLAS-1
+// BLAS-1.  Fields are processed in 16-byte packs (2 doubles, 4 floats or 8
+// 16-bit values per load) so that every precision streams at full width.
 // ---------------------------------------------------------------------------
+template <typename T> struct Vec;
+template <> struct Vec<double> {
+  static constexpr int N = 2;
+  static __device__ __forceinline__ void ld(const double* p, double* o) {
+    const double2 v = *reinterpret_cast<const double2*>(p);
+    o[0] = v.x; o[1] = v.y;
+  }
+  static __device__ __forceinline__ void st(double* p, const double* o) {
+    *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
+  }
+};
+template <> struct Vec<float> {
+  static constexpr int N = 4;
+  static __device__ __forceinline__ void ld(const float* p, float* o) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  }
+  static __device__ __forceinline__ void st(float* p, const float* o) {
+    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+  }
+};
+template <> struct Vec<__half> {
+  static constexpr int N = 8;
+  static __device__ __forceinline__ void ld(const __half* p, float* o) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      __half2 h = *reinterpret_cast<const __half2*>(&w[i]);
+      float2 f = __half22float2(h);
+      o[2 * i] = f.x; o[2 * i + 1] = f.y;
+    }
+  }
+  static __device__ __forceinline__ void st(__half* p, const float* o) {
+    unsigned w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      __half2 h = __floats2half2_rn(o[2 * i], o[2 * i + 1]);
+      w[i] = *reinterpret_cast<const unsigned*>(&h);
+    }
+    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+};
+template <> struct Vec<__hip_bfloat16> {
+  static constexpr int N = 8;
+  static __device__ __forceinline__ void ld(const __hip_bfloat16* p, float* o) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      o[2 * i] = __uint_as_float(w[i] << 16);
+      o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+    }
+  }
+  static __device__ __forceinline__ void st(__hip_bfloat16* p, const float* o) {
+    unsigned w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      __hip_bfloat16 a = __float2bfloat16(o[2 * i]), b = __float2bfloat16(o[2 * i + 1]);
+      w[i] = (unsigned)(*reinterpret_cast<const unsigned short*>(&a)) |
+             ((unsigned)(*reinterpret_cast<const unsigned short*>(&b)) << 16);
+    }
+    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+};
+
+// grid-stride over packs; the scalar tail (n % N) is done by thread 0 of block 0
+#define PACK_LOOP(T, n)                                                            \
+  constexpr int NV = Vec<T>::N;                                                    \
+  const size_t npk = (n) / NV;                                                     \
+  for (size_t ip = blockIdx.x * (size_t)blockDim.x + threadIdx.x; ip < npk;        \
+       ip += (size_t)gridDim.x * blockDim.x)
+#define TAIL_LOOP(n)                                                               \
+  if (blockIdx.x == 0 && threadIdx.x == 0)                                         \
+    for (size_t i = npk * NV; i < (n); ++i)
+
 template <typename T>
-__global__ void k_set_zero(T* __restrict__ x, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    St<T>::st(x + i, 0);
+__global__ void __launch_bounds__(256) k_set_zero(T* __restrict__ x, size_t n) {
+  using C = typename St<T>::C;
+  C z[Vec<T>::N] = {};
+  PACK_LOOP(T, n) Vec<T>::st(x + ip * NV, z);
+  TAIL_LOOP(n) St<T>::st(x + i, 0);
 }
 template <typename T>
-__global__ void k_copy(T* __restrict__ y, const T* __restrict__ x, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+__global__ void __launch_bounds__(256) k_copy(T* __restrict__ y, const T* __restrict__ x, size_t n) {
+  const size_t nb = n * sizeof(T) / 16;
+  const uint4* xs = reinterpret_cast<const uint4*>(x);
+  uint4* ys = reinterpret_cast<uint4*>(y);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nb;
        i += (size_t)gridDim.x * blockDim.x)
-    y[i] = x[i];
+    ys[i] = xs[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (size_t i = nb * 16 / sizeof(T); i < n; ++i) y[i] = x[i];
 }
 template <typename T>
-__global__ void k_axpy(T* __restrict__ y, typename St<T>::C a, const T* __restrict__ x, size_t n) {
-  using S = St<T>;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    S::st(y + i, S::ld(y + i) + a * S::ld(x + i));
+__global__ void __launch_bounds__(256)
+k_axpy(T* __restrict__ y, typename St<T>::C a, const T* __restrict__ x, size_t n) {
+  using C = typename St<T>::C;
+  PACK_LOOP(T, n) {
+    C xv[NV], yv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(y + ip * NV, yv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) yv[j] += a * xv[j];
+    Vec<T>::st(y + ip * NV, yv);
+  }
+  TAIL_LOOP(n) St<T>::st(y + i, St<T>::ld(y + i) + a * St<T>::ld(x + i));
 }
 template <typename T>
-__global__ void k_aypx(typename St<T>::C a, T* __restrict__ y, const T* __restrict__ x, size_t n) {
-  using S = St<T>;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    S::st(y + i, a * S::ld(y + i) + S::ld(x + i));
+__global__ void __launch_bounds__(256)
+k_aypx(typename St<T>::C a, T* __restrict__ y, const T* __restrict__ x, size_t n) {
+  using C = typename St<T>::C;
+  PACK_LOOP(T, n) {
+    C xv[NV], yv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(y + ip * NV, yv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) yv[j] = a * yv[j] + xv[j];
+    Vec<T>::st(y + ip * NV, yv);
+  }
+  TAIL_LOOP(n) St<T>::st(y + i, a * St<T>::ld(y + i) + St<T>::ld(x + i));
 }
 template <typename T>
-__global__ void k_scal(T* __restrict__ x, typename St<T>::C a, size_t n) {
-  using S = St<T>;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    S::st(x + i, a * S::ld(x + i));
+__global__ void __launch_bounds__(256) k_scal(T* __restrict__ x, typename St<T>::C a, size_t n) {
+  using C = typename St<T>::C;
+  PACK_LOOP(T, n) {
+    C xv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) xv[j] *= a;
+    Vec<T>::st(x + ip * NV, xv);
+  }
+  TAIL_LOOP(n) St<T>::st(x + i, a * St<T>::ld(x + i));
 }
 template <typename TO, typename TI>
-__global__ void k_convert(TO* __restrict__ y, const TI* __restrict__ x, size_t n) {
+__global__ void __launch_bounds__(256) k_convert(TO* __restrict__ y, const TI* __restrict__ x, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     St<TO>::st(y + i, (typename St<TO>::C)St<TI>::ld(x + i));
@@ -687,12 +865,26 @@ static dim3 red_grid(size_t n) {
   return dim3((unsigned)g);
 }
 
+// value of v after a round trip through the storage type
+template <typename T>
+__device__ __forceinline__ typename St<T>::C rounded(typename St<T>::C v) {
+  T t;
+  St<T>::st(&t, v);
+  return St<T>::ld(&t);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_norm2(const T* __restrict__ x, size_t n, double* partial, unsigned* counter, double* result) {
+  using C = typename St<T>::C;
   double v[1] = {0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
+  PACK_LOOP(T, n) {
+    C a[NV];
+    Vec<T>::ld(x + ip * NV, a);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[0] += (double)a[j] * (double)a[j];
+  }
+  TAIL_LOOP(n) {
     double a = (double)St<T>::ld(x + i);
     v[0] += a * a;
   }
@@ -702,10 +894,16 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_dot(const T* __restrict__ x, const T* __restrict__ y, size_t n, double* partial,
       unsigned* counter, double* result) {
+  using C = typename St<T>::C;
   double v[1] = {0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    v[0] += (double)St<T>::ld(x + i) * (double)St<T>::ld(y + i);
+  PACK_LOOP(T, n) {
+    C a[NV], b[NV];
+    Vec<T>::ld(x + ip * NV, a);
+    Vec<T>::ld(y + ip * NV, b);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[0] += (double)a[j] * (double)b[j];
+  }
+  TAIL_LOOP(n) v[0] += (double)St<T>::ld(x + i) * (double)St<T>::ld(y + i);
   block_finish<1>(v, partial, counter, result);
 }
 // complex <x, y> with the device layout: real and imaginary parts are the
@@ -727,48 +925,84 @@ k_dotc(const T* __restrict__ x, const T* __restrict__ y, size_t npairs, int Ls, 
   }
   block_finish<2>(v, partial, counter, result);
 }
+// x += a p;  r -= a s;  |r|^2 of the stored r
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_cg_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ s,
         typename St<T>::C a, size_t n, double* partial, unsigned* counter, double* result) {
-  using S = St<T>;
-  using C = typename S::C;
+  using C = typename St<T>::C;
   double v[1] = {0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    S::st(x + i, S::ld(x + i) + a * S::ld(p + i));
-    C rn = S::ld(r + i) - a * S::ld(s + i);
-    S::st(r + i, rn);
-    double rs = (double)S::ld(r + i);
-    v[0] += rs * rs;
+  PACK_LOOP(T, n) {
+    C xv[NV], rv[NV], pv[NV], sv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(p + ip * NV, pv);
+    Vec<T>::ld(r + ip * NV, rv);
+    Vec<T>::ld(s + ip * NV, sv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      xv[j] += a * pv[j];
+      rv[j] = rounded<T>(rv[j] - a * sv[j]);
+      v[0] += (double)rv[j] * (double)rv[j];
+    }
+    Vec<T>::st(x + ip * NV, xv);
+    Vec<T>::st(r + ip * NV, rv);
+  }
+  TAIL_LOOP(n) {
+    St<T>::st(x + i, St<T>::ld(x + i) + a * St<T>::ld(p + i));
+    C rn = rounded<T>(St<T>::ld(r + i) - a * St<T>::ld(s + i));
+    St<T>::st(r + i, rn);
+    v[0] += (double)rn * (double)rn;
   }
   block_finish<1>(v, partial, counter, result);
 }
+// y += a x;  Re<y, x> of the stored y
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_axpy_dot(T* __restrict__ y, typename St<T>::C a, const T* __restrict__ x, size_t n,
            double* partial, unsigned* counter, double* result) {
-  using S = St<T>;
+  using C = typename St<T>::C;
   double v[1] = {0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const auto xv = S::ld(x + i);
-    S::st(y + i, S::ld(y + i) + a * xv);
-    v[0] += (double)S::ld(y + i) * (double)xv;
+  PACK_LOOP(T, n) {
+    C xv[NV], yv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(y + ip * NV, yv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      yv[j] = rounded<T>(yv[j] + a * xv[j]);
+      v[0] += (double)yv[j] * (double)xv[j];
+    }
+    Vec<T>::st(y + ip * NV, yv);
+  }
+  TAIL_LOOP(n) {
+    const C xv = St<T>::ld(x + i);
+    const C yv = rounded<T>(St<T>::ld(y + i) + a * xv);
+    St<T>::st(y + i, yv);
+    v[0] += (double)yv * (double)xv;
   }
   block_finish<1>(v, partial, counter, result);
 }
+// r = ax - b;  |r|^2 of the stored r
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_sub_norm2(T* __restrict__ r, const T* __restrict__ ax, const T* __restrict__ b, size_t n,
             double* partial, unsigned* counter, double* result) {
-  using S = St<T>;
+  using C = typename St<T>::C;
   double v[1] = {0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    S::st(r + i, S::ld(ax + i) - S::ld(b + i));
-    double t = (double)S::ld(r + i);
-    v[0] += t * t;
+  PACK_LOOP(T, n) {
+    C av[NV], bv[NV];
+    Vec<T>::ld(ax + ip * NV, av);
+    Vec<T>::ld(b + ip * NV, bv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      av[j] = rounded<T>(av[j] - bv[j]);
+      v[0] += (double)av[j] * (double)av[j];
+    }
+    Vec<T>::st(r + ip * NV, av);
+  }
+  TAIL_LOOP(n) {
+    const C t = rounded<T>(St<T>::ld(ax + i) - St<T>::ld(b + i));
+    St<T>::st(r + i, t);
+    v[0] += (double)t * (double)t;
   }
   block_finish<1>(v, partial, counter, result);
 }
@@ -855,8 +1089,21 @@ k_shift_update(double* const* __restrict__ x, double* const* __restrict__ p,
   double* pj = p[j];
   const double c0 = coef[3 * j], c1 = coef[3 * j + 1], c2 = coef[3 * j + 2];
   double v[1] = {0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
+  PACK_LOOP(double, n) {
+    double xv[2], pv[2], rv[2];
+    Vec<double>::ld(xj + ip * 2, xv);
+    Vec<double>::ld(pj + ip * 2, pv);
+    Vec<double>::ld(r + ip * 2, rv);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      xv[k] += c0 * pv[k];
+      pv[k] = c1 * pv[k] + c2 * rv[k];
+      v[0] += pv[k] * pv[k];
+    }
+    Vec<double>::st(xj + ip * 2, xv);
+    Vec<double>::st(pj + ip * 2, pv);
+  }
+  TAIL_LOOP(n) {
     const double pv = pj[i];
     xj[i] += c0 * pv;
     const double np = c1 * pv + c2 * r[i];
