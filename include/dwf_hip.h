@@ -168,6 +168,35 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond,
                              const void* b, int* nconv, double* diff);
 
 /* ------------------------------------------------------------------------
+ * DWF fermion force, FP64: AForce_F_Domainwall_5din_eo
+ * (src/lib_alt_QXS/Force/Fermion/aforce_F_Domainwall_5din_eo-tmpl.h:299-451,
+ *  lexical part aforce_F_Domainwall_5din-tmpl.h:338-933, tensorProd_5din
+ *  afield_Spinor-inc.h:322-370, mult_generator aforce_F.cpp:97-139).
+ * Force fields are device arrays in the Field_G (18, V, 4) lexical layout.
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_force_s* dwf_force;
+
+int dwf_force_create(dwf_fopr op, dwf_force* out);
+int dwf_force_destroy(dwf_force f);
+/* set_mode: D, Ddag, H or Hdag (the mode of force_udiv1) */
+int dwf_force_set_mode(dwf_force f, const char* mode);
+/* force_udiv1(F, zeta, eta) in the current mode; F overwritten */
+int dwf_force_udiv1(dwf_force f, double* F, const void* zeta, const void* eta);
+/* force_udiv(F, eta) = udiv1_D(D eta, eta) + udiv1_Ddag(eta, D eta) */
+int dwf_force_udiv(dwf_force f, double* F, const void* eta);
+/* mult_generator: F <- -2 at(U F) */
+int dwf_force_mult_generator(dwf_force f, double* F);
+/* force_core = mult_generator(force_udiv), force_core1 = mult_generator(force_udiv1) */
+int dwf_force_core(dwf_force f, double* F, const void* eta);
+int dwf_force_core1(dwf_force f, double* F, const void* zeta, const void* eta);
+
+/* Plain device memory helpers (synchronous copies on the library stream). */
+int dwf_device_alloc(size_t bytes, void** ptr);
+int dwf_device_free(void* ptr);
+int dwf_memcpy_to_device(void* dst_device, const void* src_host, size_t bytes);
+int dwf_memcpy_to_host(void* dst_host, const void* src_device, size_t bytes);
+
+/* ------------------------------------------------------------------------
  * Stout smearing: Director_alt_Smear + ASmear_APE + AProjection_Stout_SU3
  * (src/lib_alt/Smear/director_alt_Smear-tmpl.h:19-88,273-286,
  *  src/lib_alt_QXS/Smear/asmear_APE-tmpl.h:183-223,

@@ -218,6 +218,7 @@ enum MatId {
   M_DEODAG,     // Mp^T k
   M_DEE, M_DEET, M_DI,
   M_G5, M_G5R,
+  M_M5F,        // -1/2 Mp: M5 weight of the force (aforce_F_Domainwall_5din-tmpl.h:355-365)
   M_COUNT
 };
 
@@ -348,6 +349,7 @@ static int build_mats(dwf_fopr_s* op) {
   }
   ms[M_G5] = G5;
   ms[M_G5R] = G5R;
+  ms[M_M5F] = cm_scale(Mp, -0.5L);
   return upload_mats(op, ms);
 }
 
@@ -1155,6 +1157,168 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
   release();
   return rc;
 }
+
+// ---- DWF fermion force (FP64) -----------------------------------------------
+// AForce_F_Domainwall_5din_eo (aforce_F_Domainwall_5din_eo-tmpl.h:299-451) with
+// the lexical tensor product of aforce_F_Domainwall_5din-tmpl.h:338-933 applied
+// to the two parity fields.
+struct dwf_force_s {
+  dwf_fopr op = nullptr;
+  std::string mode = "D";
+  void* w[8] = {};
+};
+
+static int force_field(dwf_force_s* f, int i, void** out) {
+  if (!f->w[i]) HIPCHK(hipMalloc(&f->w[i], f->op->nelem * sizeof(double)));
+  *out = f->w[i];
+  return DWF_OK;
+}
+
+// F = -(sum of tensor products) for udiv1_D (dag = false) or udiv1_Ddag
+static int force_udiv1_eo(dwf_force_s* f, double* F, const void* zeta, const void* eta, bool dag,
+                          bool accumulate) {
+  dwf_fopr op = f->op;
+  void *a_o, *b_e, *b_o, *t, *y_e, *y_o;
+  RC(force_field(f, 0, &a_o));
+  RC(force_field(f, 1, &b_e));
+  RC(force_field(f, 2, &b_o));
+  RC(force_field(f, 3, &t));
+  RC(force_field(f, 4, &y_e));
+  RC(force_field(f, 5, &y_o));
+  const void* plain = dag ? zeta : eta;   // (x, Doo^-1 Doe x)
+  const void* dside = dag ? eta : zeta;   // (Dee^-+ x, Doo^-+ Deo^+ Dee^-+ x)
+  RC(mult_impl(op, t, plain, "Doe", false));
+  RC(mult_impl(op, a_o, t, "Doo_inv", false));
+  RC(mult_impl(op, b_e, dside, "Dee_inv", true));
+  RC(mult_impl(op, t, b_e, "Deo", true));
+  RC(mult_impl(op, b_o, t, "Doo_inv", true));
+  RC(run_site(op, y_e, plain, M_M5F));
+  RC(run_site(op, y_o, a_o, M_M5F));
+  const int* N = op->N;
+  if (!dag) {  // W = (1 - g_mu) y(x + mu), Z = dagger side
+    HIPCHK(launch_force_tensor(F, (const double*)b_e, (const double*)y_o, op->nbr, 0, N[0], N[1],
+                               N[2], N[3], op->bc, op->Ls, -1.0, -1.0, accumulate, g_ctx.stream));
+    HIPCHK(launch_force_tensor(F, (const double*)b_o, (const double*)y_e, op->nbr, 1, N[0], N[1],
+                               N[2], N[3], op->bc, op->Ls, -1.0, -1.0, accumulate, g_ctx.stream));
+  } else {     // W = (1 + g_mu) eta'(x + mu), Z = M5-weighted zeta
+    HIPCHK(launch_force_tensor(F, (const double*)y_e, (const double*)b_o, op->nbr, 0, N[0], N[1],
+                               N[2], N[3], op->bc, op->Ls, +1.0, -1.0, accumulate, g_ctx.stream));
+    HIPCHK(launch_force_tensor(F, (const double*)y_o, (const double*)b_e, op->nbr, 1, N[0], N[1],
+                               N[2], N[3], op->bc, op->Ls, +1.0, -1.0, accumulate, g_ctx.stream));
+  }
+  return DWF_OK;
+}
+
+static int force_udiv1_mode(dwf_force_s* f, double* F, const void* zeta, const void* eta,
+                            bool accumulate) {
+  const std::string& m = f->mode;
+  if (m == "D") return force_udiv1_eo(f, F, zeta, eta, false, accumulate);
+  if (m == "Ddag") return force_udiv1_eo(f, F, zeta, eta, true, accumulate);
+  void* g;
+  RC(force_field(f, 6, &g));
+  if (m == "H") {
+    RC(run_site(f->op, g, zeta, M_G5R));
+    return force_udiv1_eo(f, F, g, eta, false, accumulate);
+  }
+  if (m == "Hdag") {
+    RC(run_site(f->op, g, eta, M_G5R));
+    return force_udiv1_eo(f, F, zeta, g, true, accumulate);
+  }
+  return fail(DWF_ERR_UNKNOWN_MODE, "irrelevant force mode: " + m);
+}
+
+extern "C" {
+
+int dwf_device_alloc(size_t bytes, void** p) {
+  CTX();
+  if (!p) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipMalloc(p, bytes));
+  return DWF_OK;
+}
+int dwf_device_free(void* p) {
+  if (!p) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  HIPCHK(hipFree(p));
+  return DWF_OK;
+}
+int dwf_memcpy_to_device(void* dst, const void* src, size_t bytes) {
+  CTX();
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_memcpy_to_host(void* dst, const void* src, size_t bytes) {
+  CTX();
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_force_create(dwf_fopr op, dwf_force* out) {
+  CTX();
+  if (!op || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (op->prec != P64) return fail(DWF_ERR_UNSUPPORTED, "the force runs on the FP64 operator");
+  if (op->alpha != 1.0) return fail(DWF_ERR_UNSUPPORTED, "the force needs parameter_alpha = 1");
+  // mult_generator recovers U from the packed forward links by their +-1 boundary factor
+  for (int mu = 0; mu < 4; ++mu)
+    if (op->bc[mu] != 1 && op->bc[mu] != -1)
+      return fail(DWF_ERR_UNSUPPORTED, "the force needs boundary conditions of +-1");
+  dwf_force_s* f = new dwf_force_s;
+  f->op = op;
+  *out = f;
+  return DWF_OK;
+}
+int dwf_force_destroy(dwf_force f) {
+  if (!f) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  for (auto& w : f->w)
+    if (w) (void)hipFree(w);
+  delete f;
+  return DWF_OK;
+}
+int dwf_force_set_mode(dwf_force f, const char* mode) {
+  if (!f || !mode) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  std::string m(mode);
+  if (m != "D" && m != "Ddag" && m != "H" && m != "Hdag")
+    return fail(DWF_ERR_UNKNOWN_MODE, "irrelevant force mode: " + m);
+  f->mode = m;
+  return DWF_OK;
+}
+int dwf_force_udiv1(dwf_force f, double* F, const void* zeta, const void* eta) {
+  CTX();
+  if (!f || !F || !zeta || !eta) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (!f->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  return force_udiv1_mode(f, F, zeta, eta, false);
+}
+int dwf_force_udiv(dwf_force f, double* F, const void* eta) {
+  CTX();
+  if (!f || !F || !eta) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (!f->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  void* z;
+  RC(force_field(f, 7, &z));
+  RC(mult_impl(f->op, z, eta, "D", false));
+  RC(force_udiv1_eo(f, F, z, eta, false, false));
+  RC(force_udiv1_eo(f, F, eta, z, true, true));
+  return DWF_OK;
+}
+int dwf_force_mult_generator(dwf_force f, double* F) {
+  CTX();
+  if (!f || !F) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  const int* N = f->op->N;
+  HIPCHK(launch_mult_generator(F, (const double*)f->op->links, N[0], N[1], N[2], N[3], f->op->bc,
+                               g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_force_core(dwf_force f, double* F, const void* eta) {
+  RC(dwf_force_udiv(f, F, eta));
+  return dwf_force_mult_generator(f, F);
+}
+int dwf_force_core1(dwf_force f, double* F, const void* zeta, const void* eta) {
+  RC(dwf_force_udiv1(f, F, zeta, eta));
+  return dwf_force_mult_generator(f, F);
+}
+
+}  // extern "C"
 
 // ---- smearing ---------------------------------------------------------------
 struct dwf_smear_s {

@@ -89,6 +89,16 @@ hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
                                const double* coef, int m, size_t n, double* partial,
                                unsigned* counters, double* pp, hipStream_t st);
 
+// Force: F_mu(x) (+)= scale * sum_{s,d} [(1 + sgn g_mu) Vn(x+mu) bc]_{a,d,s} conj(Z(x)_{b,d,s})
+// for the sites x of one parity; F lexical Field_G (4, V, 18) double.
+hipError_t launch_force_tensor(double* F, const double* Z, const double* Vn, const int* nbr,
+                               int parity, int Nx, int Ny, int Nz, int Nt, const int* bc,
+                               int Ls, double sgn, double scale, bool accumulate,
+                               hipStream_t st);
+// F <- -2 at(U F) with U from the packed forward links
+hipError_t launch_mult_generator(double* F, const double* links, int Nx, int Ny, int Nz, int Nt,
+                                 const int* bc, hipStream_t st);
+
 // Gauge: one stout smearing level, lexical Field_G (4, V, 18) double device.
 hipError_t launch_stout(double* Uout, const double* Uin, int Nx, int Ny, int Nz, int Nt,
                         double rho, hipStream_t st);

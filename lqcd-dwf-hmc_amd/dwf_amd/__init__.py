@@ -308,6 +308,83 @@ class AShiftsolver_CG:
         return n.value, float(d[0])
 
 
+class DeviceArray:
+    """A plain device buffer of float64 (e.g. a Field_G force field)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = C.c_void_p()
+        _lib.check("dwf_device_alloc", _lib.load().dwf_device_alloc(self.nbytes, C.byref(p)))
+        self._p = p.value
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def to_host(self):
+        out = np.empty(self.shape)
+        _lib.check("dwf_memcpy_to_host", _lib.load().dwf_memcpy_to_host(
+            out.ctypes.data_as(C.c_void_p), self._p, self.nbytes))
+        return out
+
+    def from_host(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size * 8 == self.nbytes
+        _lib.check("dwf_memcpy_to_device", _lib.load().dwf_memcpy_to_device(
+            self._p, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            try:
+                _lib.load().dwf_device_free(self._p)
+            except Exception:
+                pass
+            self._p = None
+
+
+class AForce_F_Domainwall_5din_eo:
+    """DWF fermion force on the GPU (FP64): force_udiv1 / force_udiv /
+    force_core / force_core1 into a device Field_G (4, V, 18)."""
+
+    def __init__(self, fopr):
+        h = C.c_void_p()
+        _lib.check("dwf_force_create", _lib.load().dwf_force_create(fopr._h, C.byref(h)))
+        self._h = h.value
+        self.fopr = fopr
+        V = int(np.prod(fopr.lattice_size))
+        self.gauge_shape = (4, V, 18)
+
+    def new_force(self):
+        return DeviceArray(self.gauge_shape)
+
+    def set_mode(self, mode):
+        _lib.check("dwf_force_set_mode", _lib.load().dwf_force_set_mode(self._h, mode.encode()))
+
+    def force_udiv1(self, F, zeta, eta):
+        _lib.check("dwf_force_udiv1", _lib.load().dwf_force_udiv1(self._h, F.ptr, zeta.ptr, eta.ptr))
+
+    def force_udiv(self, F, eta):
+        _lib.check("dwf_force_udiv", _lib.load().dwf_force_udiv(self._h, F.ptr, eta.ptr))
+
+    def mult_generator(self, F):
+        _lib.check("dwf_force_mult_generator", _lib.load().dwf_force_mult_generator(self._h, F.ptr))
+
+    def force_core(self, F, eta):
+        _lib.check("dwf_force_core", _lib.load().dwf_force_core(self._h, F.ptr, eta.ptr))
+
+    def force_core1(self, F, zeta, eta):
+        _lib.check("dwf_force_core1", _lib.load().dwf_force_core1(self._h, F.ptr, zeta.ptr, eta.ptr))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.load().dwf_force_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
 class Director_alt_Smear:
     """Stout smearing levels (APE staples + stout projection) on the GPU."""
 

@@ -91,6 +91,18 @@ _SIGS = {
     "dwf_solver_destroy": (C.c_int, [_vp]),
     "dwf_shiftsolver_cg_solve": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _dp,
                                            C.POINTER(_vp), _vp, _ip, _dp]),
+    "dwf_force_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "dwf_force_destroy": (C.c_int, [_vp]),
+    "dwf_force_set_mode": (C.c_int, [_vp, C.c_char_p]),
+    "dwf_force_udiv1": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dwf_force_udiv": (C.c_int, [_vp, _vp, _vp]),
+    "dwf_force_mult_generator": (C.c_int, [_vp, _vp]),
+    "dwf_force_core": (C.c_int, [_vp, _vp, _vp]),
+    "dwf_force_core1": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dwf_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),
+    "dwf_device_free": (C.c_int, [_vp]),
+    "dwf_memcpy_to_device": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "dwf_memcpy_to_host": (C.c_int, [_vp, _vp, C.c_size_t]),
     "dwf_smear_create": (C.c_int, [_ip, C.c_double, C.c_int, C.POINTER(_vp)]),
     "dwf_smear_destroy": (C.c_int, [_vp]),
     "dwf_smear_set_config": (C.c_int, [_vp, _dp]),

@@ -899,6 +899,137 @@ struct GaugeGeom {
   }
 };
 
+// ---- DWF fermion force (aforce_F_Domainwall_5din_eo-tmpl.h:299-451 and the
+// lexical aforce_F_Domainwall_5din-tmpl.h:338-933), restated on the two parity
+// fields instead of a lexical 5D field.
+
+// y = -1/2 [b_s w_s + c_s (P- w_{s+1} + P+ w_{s-1})], the wall terms times -mq
+// (aforce_F_Domainwall_5din-tmpl.h:355-365: b1 = -b/2, Fup = Fdn = -c/4 with 1 -/+ g5)
+void m5_force_weight(Dwf<double>& d, double* y, const double* w) {
+  const int Ls = d.Ls;
+  const long Vh = d.g.Vh;
+#pragma omp parallel for schedule(static)
+  for (long s2 = 0; s2 < Vh; ++s2) {
+    for (int s = 0; s < Ls; ++s) {
+      const int su = (s + 1) % Ls, sd = (s - 1 + Ls) % Ls;
+      const double fup = (s == Ls - 1) ? -d.mq : 1.0;
+      const double fdn = (s == 0) ? -d.mq : 1.0;
+      const double* ws = w + 24 * (s2 + Vh * s);
+      const double* wu = w + 24 * (s2 + Vh * su);
+      const double* wd = w + 24 * (s2 + Vh * sd);
+      double* ys = y + 24 * (s2 + Vh * s);
+      for (int k = 0; k < 12; ++k) {
+        // P- v = (v - g5 v)/2, P+ v = (v + g5 v)/2; g5 exchanges k and k + 12
+        const double pm_up = 0.5 * (wu[k] - wu[k + 12]);     // upper half of P- w_{s+1}
+        const double pp_dn = 0.5 * (wd[k] + wd[k + 12]);     // upper half of P+ w_{s-1}
+        const double mv_hi = fup * pm_up + fdn * pp_dn;
+        const double mv_lo = -fup * pm_up + fdn * pp_dn;      // lower halves: P- -> -, P+ -> +
+        ys[k] = -0.5 * (d.b[s] * ws[k] + d.c[s] * mv_hi);
+        ys[k + 12] = -0.5 * (d.b[s] * ws[k + 12] + d.c[s] * mv_lo);
+      }
+    }
+  }
+}
+
+// v <- (1 + sgn gamma_mu) v for one 4-spinor (Dirac basis, gammaMatrixSet_Dirac.cpp:37-65)
+inline void one_pm_gamma(double* out, const double* v, int mu, int sgn) {
+  cd x[4][3], g[4][3];
+  for (int dd = 0; dd < 4; ++dd)
+    for (int c = 0; c < 3; ++c) x[dd][c] = cd(v[2 * c + 6 * dd], v[2 * c + 6 * dd + 1]);
+  const cd I(0, 1);
+  for (int c = 0; c < 3; ++c) {
+    if (mu == 0) { g[0][c] = -I * x[3][c]; g[1][c] = -I * x[2][c]; g[2][c] = I * x[1][c]; g[3][c] = I * x[0][c]; }
+    else if (mu == 1) { g[0][c] = -x[3][c]; g[1][c] = x[2][c]; g[2][c] = x[1][c]; g[3][c] = -x[0][c]; }
+    else if (mu == 2) { g[0][c] = -I * x[2][c]; g[1][c] = I * x[3][c]; g[2][c] = I * x[0][c]; g[3][c] = -I * x[1][c]; }
+    else { g[0][c] = x[0][c]; g[1][c] = x[1][c]; g[2][c] = -x[2][c]; g[3][c] = -x[3][c]; }
+  }
+  for (int dd = 0; dd < 4; ++dd)
+    for (int c = 0; c < 3; ++c) {
+      const cd r = x[dd][c] + (double)sgn * g[dd][c];
+      out[2 * c + 6 * dd] = r.real();
+      out[2 * c + 6 * dd + 1] = r.imag();
+    }
+}
+
+// F_mu(x) += sum_{s,d} W(x)_{a,d,s} conj(Z(x)_{b,d,s}),  W(x) = (1 + sgn g_mu) V(x+mu)
+// times the boundary factor when x+mu wraps (tensorProd_5din, afield_Spinor-inc.h:322-370,
+// after the backward shift with boundary conditions).  x runs over parity p.
+void tensor_fwd(Dwf<double>& d, double* F, const double* Z, const double* Vn, int p, int sgn) {
+  const int Ls = d.Ls;
+  const long Vh = d.g.Vh, V = d.g.V;
+#pragma omp parallel for schedule(static)
+  for (long s2 = 0; s2 < Vh; ++s2) {
+    for (int mu = 0; mu < 4; ++mu) {
+      const int nb = d.nbr[p][mu][0][s2];
+      const double bcv = d.bcf[p][mu][0][s2];
+      const long lx = d.lnk[p][mu][0][s2];
+      cd acc[3][3] = {};
+      for (int s = 0; s < Ls; ++s) {
+        double w[24];
+        one_pm_gamma(w, Vn + 24 * (nb + Vh * s), mu, sgn);
+        const double* z = Z + 24 * (s2 + Vh * s);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            for (int dd = 0; dd < 4; ++dd)
+              acc[a][b] += cd(w[2 * a + 6 * dd], w[2 * a + 6 * dd + 1]) *
+                           std::conj(cd(z[2 * b + 6 * dd], z[2 * b + 6 * dd + 1]));
+      }
+      double* f = F + 18 * (lx + V * mu);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          f[2 * (3 * a + b)] += bcv * acc[a][b].real();
+          f[2 * (3 * a + b) + 1] += bcv * acc[a][b].imag();
+        }
+    }
+  }
+}
+
+// force_udiv1_D / force_udiv1_Ddag of the even-odd operator; output -F (the
+// eo wrapper's sign), F lexical Field_G (18, V, 4).
+void force_udiv1_eo(Dwf<double>& d, double* F, const double* zeta, const double* eta, bool dag) {
+  const long n = n5(d), ng = 18L * d.g.V * 4;
+  std::vector<double> a_e(n), a_o(n), b_e(n), b_o(n), t(n), y_e(n), y_o(n);
+  // "plain" side: (x_e, Doo^-1 Doe x_e); "dagger" side: (Dee^-+ x, Doo^-+ Deo^+ Dee^-+ x)
+  const double* plain = dag ? zeta : eta;
+  const double* dside = dag ? eta : zeta;
+  std::copy(plain, plain + n, a_e.begin());
+  mult_mode(d, "Doe", t.data(), plain);
+  mult_mode(d, "Doo_inv", a_o.data(), t.data());
+  mult_dag_mode(d, "Dee_inv", b_e.data(), dside);
+  mult_dag_mode(d, "Deo", t.data(), b_e.data());
+  mult_dag_mode(d, "Doo_inv", b_o.data(), t.data());
+  std::fill(F, F + ng, 0.0);
+  if (!dag) {
+    // W = (1 - g_mu) y(x + mu) with y the M5-weighted eta, Z = dagger-side zeta
+    m5_force_weight(d, y_e.data(), a_e.data());
+    m5_force_weight(d, y_o.data(), a_o.data());
+    tensor_fwd(d, F, b_e.data(), y_o.data(), 0, -1);
+    tensor_fwd(d, F, b_o.data(), y_e.data(), 1, -1);
+  } else {
+    // W = (1 + g_mu) eta'(x + mu), Z = the M5-weighted zeta
+    m5_force_weight(d, y_e.data(), a_e.data());
+    m5_force_weight(d, y_o.data(), a_o.data());
+    tensor_fwd(d, F, y_e.data(), b_o.data(), 0, +1);
+    tensor_fwd(d, F, y_o.data(), b_e.data(), 1, +1);
+  }
+  for (long i = 0; i < ng; ++i) F[i] = -F[i];
+}
+
+// mult_generator (lib_alt_QXS/Force/Fermion/aforce_F.cpp:97-110): F <- -2 at(U F)
+void mult_generator(Dwf<double>& d, double* F) {
+  const long V = d.g.V;
+#pragma omp parallel for schedule(static)
+  for (long site = 0; site < V; ++site)
+    for (int mu = 0; mu < 4; ++mu) {
+      double* f = F + 18 * (site + V * mu);
+      M3 UF = mul(load3(d.U.data() + 18 * (site + V * mu)), load3(f));
+      M3 A = at3(UF);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A.a[i][j] *= -2.0;
+      store3(f, A);
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1043,6 +1174,35 @@ int orc_shiftcg_solve(orc_dwf h, const char* mode, int nshift, const double* sig
 }
 
 double orc_norm2(const double* x, long n) { return norm2v(x, n); }
+
+// AForce_F_Domainwall_5din_eo::force_udiv1 in mode D, Ddag, H or Hdag
+// (aforce_F_Domainwall_5din_eo-tmpl.h:318-451); H/Hdag apply gamma_5 R to
+// zeta / eta first (aforce_F_Domainwall_5din-tmpl.h:913-933).
+int orc_force_udiv1(orc_dwf h, const char* mode, double* F, const double* zeta,
+                    const double* eta) {
+  Dwf<double>& d = h->d64;
+  std::string m(mode);
+  if (m == "D") force_udiv1_eo(d, F, zeta, eta, false);
+  else if (m == "Ddag") force_udiv1_eo(d, F, zeta, eta, true);
+  else if (m == "H" || m == "Hdag") {
+    std::vector<double> t(n5(d));
+    if (m == "H") { gm5R(d, t.data(), zeta); force_udiv1_eo(d, F, t.data(), eta, false); }
+    else { gm5R(d, t.data(), eta); force_udiv1_eo(d, F, zeta, t.data(), true); }
+  } else return -1;
+  return 0;
+}
+// force_udiv (mode D): z = D eta; udiv1_D(z, eta) + udiv1_Ddag(eta, z)
+// (aforce_F_Domainwall_5din_eo-tmpl.h:299-315)
+void orc_force_udiv(orc_dwf h, double* F, const double* eta) {
+  Dwf<double>& d = h->d64;
+  const long ng = 18L * d.g.V * 4;
+  std::vector<double> z(n5(d)), F2(ng);
+  mult_mode(d, "D", z.data(), eta);
+  force_udiv1_eo(d, F, z.data(), eta, false);
+  force_udiv1_eo(d, F2.data(), eta, z.data(), true);
+  for (long i = 0; i < ng; ++i) F[i] += F2[i];
+}
+void orc_mult_generator(orc_dwf h, double* F) { mult_generator(h->d64, F); }
 double orc_dot(const double* x, const double* y, long n) { return dotv(x, y, n); }
 
 // One APE-staple + stout projection level (asmear_APE-tmpl.h:183-223):

@@ -71,6 +71,14 @@ int orc_shiftcg_solve(orc_dwf d, const char* mode, int nshift,
                       const double* sigma, double** x, const double* b,
                       int niter, double stop_cond2, int* nconv, double* diff);
 
+/* DWF fermion force of the even-odd operator (FP64).  F: Field_G (18, V, 4)
+ * lexical, overwritten.  mode: D, Ddag, H, Hdag.  force_core1 = udiv1 followed
+ * by orc_mult_generator (F <- -2 at(U F)). */
+int orc_force_udiv1(orc_dwf d, const char* mode, double* F, const double* zeta,
+                    const double* eta);
+void orc_force_udiv(orc_dwf d, double* F, const double* eta);
+void orc_mult_generator(orc_dwf d, double* F);
+
 /* BLAS helpers on double arrays of n reals. */
 double orc_norm2(const double* x, long n);
 double orc_dot(const double* x, const double* y, long n);

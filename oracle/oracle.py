@@ -50,6 +50,10 @@ def lib():
         L.orc_shiftcg_solve.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, C.POINTER(_dp),
                                         _dp, C.c_int, C.c_double, C.POINTER(C.c_int), _dp]
         L.orc_shiftcg_solve.restype = C.c_int
+        L.orc_force_udiv1.argtypes = [C.c_void_p, C.c_char_p, _dp, _dp, _dp]
+        L.orc_force_udiv1.restype = C.c_int
+        L.orc_force_udiv.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_mult_generator.argtypes = [C.c_void_p, _dp]
         L.orc_stout_smear.argtypes = [C.POINTER(C.c_int), C.c_double, _dp, _dp]
         L.orc_plaquette.argtypes = [C.POINTER(C.c_int), _dp]
         L.orc_plaquette.restype = C.c_double
@@ -121,6 +125,35 @@ class OracleDWF:
         v4 = np.empty_like(w4)
         lib().orc_wilson_meo(self._h, _ptr(v4), _ptr(w4), int(ieo))
         return v4
+
+    def _gauge_shape(self):
+        V = self.dims[0] * self.dims[1] * self.dims[2] * self.dims[3]
+        return (4, V, 18)
+
+    def force_udiv1(self, mode, zeta, eta):
+        zeta = np.ascontiguousarray(zeta, np.float64)
+        eta = np.ascontiguousarray(eta, np.float64)
+        F = np.empty(self._gauge_shape())
+        if lib().orc_force_udiv1(self._h, mode.encode(), _ptr(F), _ptr(zeta), _ptr(eta)) != 0:
+            raise ValueError(f"unknown force mode {mode}")
+        return F
+
+    def force_udiv(self, eta):
+        eta = np.ascontiguousarray(eta, np.float64)
+        F = np.empty(self._gauge_shape())
+        lib().orc_force_udiv(self._h, _ptr(F), _ptr(eta))
+        return F
+
+    def mult_generator(self, F):
+        F = np.ascontiguousarray(F, np.float64).copy()
+        lib().orc_mult_generator(self._h, _ptr(F))
+        return F
+
+    def force_core(self, eta):
+        return self.mult_generator(self.force_udiv(eta))
+
+    def force_core1(self, mode, zeta, eta):
+        return self.mult_generator(self.force_udiv1(mode, zeta, eta))
 
     def cg(self, b, mode="DdagD", niter=10000, stop_cond2=1e-24, init="RHS"):
         b = np.ascontiguousarray(b, np.float64)
