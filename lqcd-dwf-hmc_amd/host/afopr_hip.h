@@ -237,6 +237,57 @@ class AShiftsolver_CG {
   double stop_;
 };
 
+// Device force field in the Field_G (18, V, 4) lexical layout (FP64).
+class AForceField {
+ public:
+  explicit AForceField(const std::vector<int>& lattice_size) {
+    bytes_ = sizeof(double) * 18 * 4;
+    for (int v : lattice_size) bytes_ *= (size_t)v;
+    check(dwf_device_alloc(bytes_, &p_), "dwf_device_alloc");
+  }
+  ~AForceField() { dwf_device_free(p_); }
+  AForceField(const AForceField&) = delete;
+  AForceField& operator=(const AForceField&) = delete;
+  double* ptr() { return static_cast<double*>(p_); }
+  size_t bytes() const { return bytes_; }
+  void to_host(double* F) const { check(dwf_memcpy_to_host(F, p_, bytes_), "dwf_memcpy_to_host"); }
+
+ private:
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// AForce_F<AFIELD> of the even-odd DWF operator
+// (src/lib/Force/Fermion/aforce_F.h:55-110,
+//  src/lib_alt_QXS/Force/Fermion/aforce_F_Domainwall_5din_eo-tmpl.h:299-451).
+class AForce_F_Domainwall_5din_eo {
+ public:
+  explicit AForce_F_Domainwall_5din_eo(AFopr_Domainwall_5din_eo& op) {
+    check(dwf_force_create(op.handle(), &h_), "dwf_force_create");
+  }
+  ~AForce_F_Domainwall_5din_eo() { dwf_force_destroy(h_); }
+  AForce_F_Domainwall_5din_eo(const AForce_F_Domainwall_5din_eo&) = delete;
+  AForce_F_Domainwall_5din_eo& operator=(const AForce_F_Domainwall_5din_eo&) = delete;
+  void set_mode(const std::string& mode) {
+    check(dwf_force_set_mode(h_, mode.c_str()), "dwf_force_set_mode");
+  }
+  void force_udiv(AForceField& F, const AField& eta) {
+    check(dwf_force_udiv(h_, F.ptr(), eta.ptr()), "dwf_force_udiv");
+  }
+  void force_udiv1(AForceField& F, const AField& zeta, const AField& eta) {
+    check(dwf_force_udiv1(h_, F.ptr(), zeta.ptr(), eta.ptr()), "dwf_force_udiv1");
+  }
+  void force_core(AForceField& F, const AField& eta) {
+    check(dwf_force_core(h_, F.ptr(), eta.ptr()), "dwf_force_core");
+  }
+  void force_core1(AForceField& F, const AField& zeta, const AField& eta) {
+    check(dwf_force_core1(h_, F.ptr(), zeta.ptr(), eta.ptr()), "dwf_force_core1");
+  }
+
+ private:
+  dwf_force h_ = nullptr;
+};
+
 class Director_alt_Smear {
  public:
   Director_alt_Smear(const std::vector<int>& lattice_size, double rho_uniform,
