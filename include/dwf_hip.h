@@ -190,6 +190,48 @@ int dwf_force_mult_generator(dwf_force f, double* F);
 int dwf_force_core(dwf_force f, double* F, const void* eta);
 int dwf_force_core1(dwf_force f, double* F, const void* zeta, const void* eta);
 
+/* ------------------------------------------------------------------------
+ * Rational functions of A^dag A, FP64.  One rational function is
+ *   r(x) = a0 + sum_i bl[i] / (x + cl[i]),  i < np,
+ * with the coefficients of a Math_Rational file (src/lib/Tools/math_Rational.cpp:
+ * a0 = norm, bl = res, cl = pole), evaluated by multishift CG on mode DdagD.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int np;               /* number_of_poles */
+  double a0;
+  const double* bl;     /* np residues */
+  const double* cl;     /* np poles (shifts; cl[0] is the base system) */
+  int max_iter;         /* maximum_number_of_iteration of the shift solver */
+  double stop_cond;     /* convergence_criterion_squared of the shift solver */
+} dwf_rational_params;
+
+/* AFopr_Rational::mult (src/lib/Fopr/afopr_Rational-tmpl.h:181-219):
+ * v = r(A^dag A) b with A = op; sets op's mode to DdagD as the reference does.
+ * nconv, diff: those of the shift solver.  DWF_ERR_NOT_CONVERGED leaves v
+ * computed from the unconverged solutions. */
+int dwf_rational_mult(dwf_fopr op, const dwf_rational_params* r, void* v, const void* b,
+                      int* nconv, double* diff);
+
+/* AForce_F_Rational_Ratio (src/lib_alt/Force/Fermion/aforce_F_Rational_Ratio-tmpl.h:
+ * 21-459): the force of S = <phi| r1(A^dag A) r2(B^dag B) r1(A^dag A) |phi>,
+ * A = fopr1 (numerator, r1 ~ (A^dag A)^(1/4)), B = fopr2 (denominator,
+ * r2 ~ (B^dag B)^(-1/2)).  Both operators FP64 on the same lattice. */
+typedef struct dwf_ratio_force_s* dwf_ratio_force;
+
+int dwf_force_rational_ratio_create(dwf_fopr fopr1, dwf_fopr fopr2,
+                                    const dwf_rational_params* numerator,
+                                    const dwf_rational_params* denominator,
+                                    dwf_ratio_force* out);
+int dwf_force_rational_ratio_destroy(dwf_ratio_force rf);
+/* force_udiv (-tmpl.h:345-447): F = sum_i bl2[i] force_udiv^B(eta2_i)
+ *   + sum_i bl1[i] [udiv1_H(H eta1_i, zeta1_i) + udiv1_H(H zeta1_i, eta1_i)
+ *                 + udiv1_Hdag(eta1_i, H zeta1_i) + udiv1_Hdag(zeta1_i, H eta1_i)];
+ * leaves fopr1 in mode H and fopr2 in mode DdagD, as the reference does.
+ * DWF_ERR_NOT_CONVERGED if a shift solve did not converge (F still computed). */
+int dwf_force_rational_ratio_udiv(dwf_ratio_force rf, double* F, const void* eta);
+/* force_core = mult_generator(force_udiv)   (lib_alt_QXS/Force/Fermion/aforce_F.cpp:124) */
+int dwf_force_rational_ratio_core(dwf_ratio_force rf, double* F, const void* eta);
+
 /* Plain device memory helpers (synchronous copies on the library stream). */
 int dwf_device_alloc(size_t bytes, void** ptr);
 int dwf_device_free(void* ptr);

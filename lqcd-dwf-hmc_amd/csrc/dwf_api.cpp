@@ -1174,9 +1174,10 @@ static int force_field(dwf_force_s* f, int i, void** out) {
   return DWF_OK;
 }
 
-// F = -(sum of tensor products) for udiv1_D (dag = false) or udiv1_Ddag
+// F (+)= scale * udiv1_D (dag = false) or udiv1_Ddag, where udiv1 = -(sum of the
+// tensor products)
 static int force_udiv1_eo(dwf_force_s* f, double* F, const void* zeta, const void* eta, bool dag,
-                          bool accumulate) {
+                          double scale, bool accumulate) {
   dwf_fopr op = f->op;
   void *a_o, *b_e, *b_o, *t, *y_e, *y_o;
   RC(force_field(f, 0, &a_o));
@@ -1197,34 +1198,44 @@ static int force_udiv1_eo(dwf_force_s* f, double* F, const void* zeta, const voi
   const int* N = op->N;
   if (!dag) {  // W = (1 - g_mu) y(x + mu), Z = dagger side
     HIPCHK(launch_force_tensor(F, (const double*)b_e, (const double*)y_o, op->nbr, 0, N[0], N[1],
-                               N[2], N[3], op->bc, op->Ls, -1.0, -1.0, accumulate, g_ctx.stream));
+                               N[2], N[3], op->bc, op->Ls, -1.0, -scale, accumulate, g_ctx.stream));
     HIPCHK(launch_force_tensor(F, (const double*)b_o, (const double*)y_e, op->nbr, 1, N[0], N[1],
-                               N[2], N[3], op->bc, op->Ls, -1.0, -1.0, accumulate, g_ctx.stream));
+                               N[2], N[3], op->bc, op->Ls, -1.0, -scale, accumulate, g_ctx.stream));
   } else {     // W = (1 + g_mu) eta'(x + mu), Z = M5-weighted zeta
     HIPCHK(launch_force_tensor(F, (const double*)y_e, (const double*)b_o, op->nbr, 0, N[0], N[1],
-                               N[2], N[3], op->bc, op->Ls, +1.0, -1.0, accumulate, g_ctx.stream));
+                               N[2], N[3], op->bc, op->Ls, +1.0, -scale, accumulate, g_ctx.stream));
     HIPCHK(launch_force_tensor(F, (const double*)y_o, (const double*)b_e, op->nbr, 1, N[0], N[1],
-                               N[2], N[3], op->bc, op->Ls, +1.0, -1.0, accumulate, g_ctx.stream));
+                               N[2], N[3], op->bc, op->Ls, +1.0, -scale, accumulate, g_ctx.stream));
   }
   return DWF_OK;
 }
 
-static int force_udiv1_mode(dwf_force_s* f, double* F, const void* zeta, const void* eta,
-                            bool accumulate) {
-  const std::string& m = f->mode;
-  if (m == "D") return force_udiv1_eo(f, F, zeta, eta, false, accumulate);
-  if (m == "Ddag") return force_udiv1_eo(f, F, zeta, eta, true, accumulate);
+static int force_udiv1_mode(dwf_force_s* f, const std::string& m, double* F, const void* zeta,
+                            const void* eta, double scale, bool accumulate) {
+  if (m == "D") return force_udiv1_eo(f, F, zeta, eta, false, scale, accumulate);
+  if (m == "Ddag") return force_udiv1_eo(f, F, zeta, eta, true, scale, accumulate);
   void* g;
   RC(force_field(f, 6, &g));
   if (m == "H") {
     RC(run_site(f->op, g, zeta, M_G5R));
-    return force_udiv1_eo(f, F, g, eta, false, accumulate);
+    return force_udiv1_eo(f, F, g, eta, false, scale, accumulate);
   }
   if (m == "Hdag") {
     RC(run_site(f->op, g, eta, M_G5R));
-    return force_udiv1_eo(f, F, zeta, g, true, accumulate);
+    return force_udiv1_eo(f, F, zeta, g, true, scale, accumulate);
   }
   return fail(DWF_ERR_UNKNOWN_MODE, "irrelevant force mode: " + m);
+}
+
+// force_udiv (aforce_F_Domainwall_5din_eo-tmpl.h:299-315):
+// F (+)= scale [udiv1_D(D eta, eta) + udiv1_Ddag(eta, D eta)]
+static int force_udiv_impl(dwf_force_s* f, double* F, const void* eta, double scale,
+                           bool accumulate) {
+  void* z;
+  RC(force_field(f, 7, &z));
+  RC(mult_impl(f->op, z, eta, "D", false));
+  RC(force_udiv1_eo(f, F, z, eta, false, scale, accumulate));
+  return force_udiv1_eo(f, F, eta, z, true, scale, true);
 }
 
 extern "C" {
@@ -1288,18 +1299,13 @@ int dwf_force_udiv1(dwf_force f, double* F, const void* zeta, const void* eta) {
   CTX();
   if (!f || !F || !zeta || !eta) return fail(DWF_ERR_INVALID_ARG, "null argument");
   if (!f->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
-  return force_udiv1_mode(f, F, zeta, eta, false);
+  return force_udiv1_mode(f, f->mode, F, zeta, eta, 1.0, false);
 }
 int dwf_force_udiv(dwf_force f, double* F, const void* eta) {
   CTX();
   if (!f || !F || !eta) return fail(DWF_ERR_INVALID_ARG, "null argument");
   if (!f->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
-  void* z;
-  RC(force_field(f, 7, &z));
-  RC(mult_impl(f->op, z, eta, "D", false));
-  RC(force_udiv1_eo(f, F, z, eta, false, false));
-  RC(force_udiv1_eo(f, F, eta, z, true, true));
-  return DWF_OK;
+  return force_udiv_impl(f, F, eta, 1.0, false);
 }
 int dwf_force_mult_generator(dwf_force f, double* F) {
   CTX();
@@ -1316,6 +1322,218 @@ int dwf_force_core(dwf_force f, double* F, const void* eta) {
 int dwf_force_core1(dwf_force f, double* F, const void* zeta, const void* eta) {
   RC(dwf_force_udiv1(f, F, zeta, eta));
   return dwf_force_mult_generator(f, F);
+}
+
+}  // extern "C"
+
+// ---- rational functions of A^dag A and AForce_F_Rational_Ratio (FP64) ----------
+
+static void force_release(dwf_force_s* f) {
+  for (auto& w : f->w)
+    if (w) (void)hipFree(w);
+  for (auto& w : f->w) w = nullptr;
+}
+
+static int check_rational(const dwf_rational_params* r) {
+  if (!r || r->np < 1 || !r->bl || !r->cl || r->max_iter < 1 || !(r->stop_cond > 0))
+    return fail(DWF_ERR_INVALID_ARG, "invalid rational parameters");
+  return DWF_OK;
+}
+
+// v = a0 b + sum_i bl_i x_i in one pass (afopr_Rational-tmpl.h:211-215)
+static int rational_combine(dwf_fopr op, void* v, const void* b, double a0,
+                            const std::vector<void*>& x, const double* bl) {
+  const int m = (int)x.size();
+  RC(ensure_shift_scratch(m));
+  hipStream_t st = g_ctx.stream;
+  double* hc = g_ctx.host_coef;
+  void** hp = (void**)(g_ctx.host_coef + 3 * g_ctx.sh_cap);
+  HIPCHK(hipStreamSynchronize(st));   // the pinned staging area is free again
+  for (int i = 0; i < m; ++i) {
+    hc[i] = bl[i];
+    hp[i] = x[i];
+  }
+  HIPCHK(hipMemcpyAsync(g_ctx.sh_coef, hc, sizeof(double) * m, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g_ctx.sh_ptrs, hp, sizeof(void*) * m, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_lincomb(v, b, a0, g_ctx.sh_ptrs, g_ctx.sh_coef, m, op->nelem, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return DWF_OK;
+}
+
+static int alloc_fields(dwf_fopr op, std::vector<void*>& v, int n) {
+  v.assign(n, nullptr);
+  for (int i = 0; i < n; ++i) HIPCHK(hipMalloc(&v[i], op->nelem * sizeof(double)));
+  return DWF_OK;
+}
+static void free_fields(std::vector<void*>& v) {
+  for (auto& p : v)
+    if (p) (void)hipFree(p);
+  v.clear();
+}
+
+// shift solve of AFopr_Rational / AForce_F_Rational_Ratio on mode DdagD; a
+// solver that does not converge is recorded in *status and the caller goes on,
+// as the reference does (it logs Nconv and diff and continues)
+static int rational_solve(dwf_fopr op, const dwf_rational_params& r, std::vector<void*>& x,
+                          const void* b, int* nconv, double* diff, int* status) {
+  op->mode = "DdagD";
+  const int rc = dwf_shiftsolver_cg_solve(op, r.max_iter, r.stop_cond, r.np, r.cl, x.data(), b,
+                                          nconv, diff);
+  if (rc == DWF_ERR_NOT_CONVERGED) {
+    *status = rc;
+    return DWF_OK;
+  }
+  return rc;
+}
+
+struct dwf_ratio_force_s {
+  dwf_fopr op1 = nullptr, op2 = nullptr;
+  dwf_force_s f1, f2;
+  dwf_rational_params r1{}, r2{};
+  std::vector<double> bl1, cl1, bl2, cl2;
+  std::vector<void*> eta1, zeta1, eta2;
+  void* v1 = nullptr;
+  void* v2 = nullptr;
+};
+
+static void ratio_release(dwf_ratio_force_s* rf) {
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  free_fields(rf->eta1);
+  free_fields(rf->zeta1);
+  free_fields(rf->eta2);
+  if (rf->v1) (void)hipFree(rf->v1);
+  if (rf->v2) (void)hipFree(rf->v2);
+  rf->v1 = rf->v2 = nullptr;
+  force_release(&rf->f1);
+  force_release(&rf->f2);
+}
+
+// aforce_F_Rational_Ratio-tmpl.h:345-447
+static int ratio_force_udiv(dwf_ratio_force_s* rf, double* F, const void* eta) {
+  dwf_fopr op1 = rf->op1;
+  int status = DWF_OK, nconv;
+  double diff;
+  // m_eta1 = (A^dag A + cl1)^-1 eta;  m_v1 = r1(A^dag A) eta
+  RC(rational_solve(op1, rf->r1, rf->eta1, eta, &nconv, &diff, &status));
+  RC(rational_combine(op1, rf->v1, eta, rf->r1.a0, rf->eta1, rf->r1.bl));
+  // m_eta2 = (B^dag B + cl2)^-1 m_v1;  m_v2 = r2(B^dag B) m_v1
+  RC(rational_solve(rf->op2, rf->r2, rf->eta2, rf->v1, &nconv, &diff, &status));
+  RC(rational_combine(op1, rf->v2, rf->v1, rf->r2.a0, rf->eta2, rf->r2.bl));
+  // m_zeta1 = (A^dag A + cl1)^-1 m_v2
+  RC(rational_solve(op1, rf->r1, rf->zeta1, rf->v2, &nconv, &diff, &status));
+  // denominator: sum_i bl2_i force_udiv^B(eta2_i)
+  for (int i = 0; i < rf->r2.np; ++i)
+    RC(force_udiv_impl(&rf->f2, F, rf->eta2[i], rf->r2.bl[i], i > 0));
+  // numerator: the four H / Hdag terms per pole
+  op1->mode = "H";
+  for (int i = 0; i < rf->r1.np; ++i) {
+    const double b = rf->r1.bl[i];
+    RC(mult_impl(op1, rf->v1, rf->eta1[i], "H", false));
+    RC(mult_impl(op1, rf->v2, rf->zeta1[i], "H", false));
+    RC(force_udiv1_mode(&rf->f1, "H", F, rf->v1, rf->zeta1[i], b, true));
+    RC(force_udiv1_mode(&rf->f1, "H", F, rf->v2, rf->eta1[i], b, true));
+    RC(force_udiv1_mode(&rf->f1, "Hdag", F, rf->eta1[i], rf->v2, b, true));
+    RC(force_udiv1_mode(&rf->f1, "Hdag", F, rf->zeta1[i], rf->v1, b, true));
+  }
+  rf->f1.mode = "Hdag";
+  if (status != DWF_OK) return fail(status, "AForce_F_Rational_Ratio: a shift solver did not converge");
+  return DWF_OK;
+}
+
+extern "C" {
+
+int dwf_rational_mult(dwf_fopr op, const dwf_rational_params* r, void* v, const void* b,
+                      int* nconv, double* diff) {
+  CTX();
+  if (!op || !v || !b || !nconv || !diff) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  RC(check_rational(r));
+  if (op->prec != P64) return fail(DWF_ERR_UNSUPPORTED, "rational functions run on the FP64 operator");
+  if (!op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  std::vector<void*> x;
+  int rc = alloc_fields(op, x, r->np);
+  int status = DWF_OK;
+  if (rc == DWF_OK) rc = rational_solve(op, *r, x, b, nconv, diff, &status);
+  if (rc == DWF_OK) rc = rational_combine(op, v, b, r->a0, x, r->bl);
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  free_fields(x);
+  if (rc != DWF_OK) return rc;
+  if (status != DWF_OK) return fail(status, "AFopr_Rational: the shift solver did not converge");
+  return DWF_OK;
+}
+
+int dwf_force_rational_ratio_create(dwf_fopr fopr1, dwf_fopr fopr2,
+                                    const dwf_rational_params* numerator,
+                                    const dwf_rational_params* denominator,
+                                    dwf_ratio_force* out) {
+  CTX();
+  if (!fopr1 || !fopr2 || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  RC(check_rational(numerator));
+  RC(check_rational(denominator));
+  for (dwf_fopr op : {fopr1, fopr2}) {
+    if (op->prec != P64) return fail(DWF_ERR_UNSUPPORTED, "the force runs on FP64 operators");
+    if (op->alpha != 1.0) return fail(DWF_ERR_UNSUPPORTED, "the force needs parameter_alpha = 1");
+    for (int mu = 0; mu < 4; ++mu)
+      if (op->bc[mu] != 1 && op->bc[mu] != -1)
+        return fail(DWF_ERR_UNSUPPORTED, "the force needs boundary conditions of +-1");
+  }
+  for (int mu = 0; mu < 4; ++mu)
+    if (fopr1->N[mu] != fopr2->N[mu])
+      return fail(DWF_ERR_INVALID_ARG, "numerator and denominator lattices differ");
+  if (fopr1->Ls != fopr2->Ls || fopr1->nelem != fopr2->nelem)
+    return fail(DWF_ERR_INVALID_ARG, "numerator and denominator fields differ");
+  dwf_ratio_force_s* rf = new dwf_ratio_force_s;
+  rf->op1 = fopr1;
+  rf->op2 = fopr2;
+  rf->f1.op = fopr1;
+  rf->f2.op = fopr2;
+  rf->bl1.assign(numerator->bl, numerator->bl + numerator->np);
+  rf->cl1.assign(numerator->cl, numerator->cl + numerator->np);
+  rf->bl2.assign(denominator->bl, denominator->bl + denominator->np);
+  rf->cl2.assign(denominator->cl, denominator->cl + denominator->np);
+  rf->r1 = *numerator;
+  rf->r1.bl = rf->bl1.data();
+  rf->r1.cl = rf->cl1.data();
+  rf->r2 = *denominator;
+  rf->r2.bl = rf->bl2.data();
+  rf->r2.cl = rf->cl2.data();
+  int rc = alloc_fields(fopr1, rf->eta1, numerator->np);
+  if (rc == DWF_OK) rc = alloc_fields(fopr1, rf->zeta1, numerator->np);
+  if (rc == DWF_OK) rc = alloc_fields(fopr1, rf->eta2, denominator->np);
+  if (rc == DWF_OK && hipMalloc(&rf->v1, fopr1->nelem * sizeof(double)) != hipSuccess)
+    rc = fail(DWF_ERR_HIP, "device allocation failed");
+  if (rc == DWF_OK && hipMalloc(&rf->v2, fopr1->nelem * sizeof(double)) != hipSuccess)
+    rc = fail(DWF_ERR_HIP, "device allocation failed");
+  if (rc != DWF_OK) {
+    ratio_release(rf);
+    delete rf;
+    return rc;
+  }
+  *out = rf;
+  return DWF_OK;
+}
+
+int dwf_force_rational_ratio_destroy(dwf_ratio_force rf) {
+  if (!rf) return DWF_OK;
+  ratio_release(rf);
+  delete rf;
+  return DWF_OK;
+}
+
+int dwf_force_rational_ratio_udiv(dwf_ratio_force rf, double* F, const void* eta) {
+  CTX();
+  if (!rf || !F || !eta) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (!rf->op1->have_config || !rf->op2->have_config)
+    return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  return ratio_force_udiv(rf, F, eta);
+}
+
+int dwf_force_rational_ratio_core(dwf_ratio_force rf, double* F, const void* eta) {
+  const int rc = dwf_force_rational_ratio_udiv(rf, F, eta);
+  if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
+  const int* N = rf->op1->N;
+  HIPCHK(launch_mult_generator(F, (const double*)rf->op1->links, N[0], N[1], N[2], N[3],
+                               rf->op1->bc, g_ctx.stream));
+  return rc;
 }
 
 }  // extern "C"

@@ -89,6 +89,10 @@ hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
                                const double* coef, int m, size_t n, double* partial,
                                unsigned* counters, double* pp, hipStream_t st);
 
+// v = a0 b + sum_{j<m} c[j] x[j] (double; x and c device arrays)
+hipError_t launch_lincomb(void* v, const void* b, double a0, void* const* x, const double* c,
+                          int m, size_t n, hipStream_t st);
+
 // Force: F_mu(x) (+)= scale * sum_{s,d} [(1 + sgn g_mu) Vn(x+mu) bc]_{a,d,s} conj(Z(x)_{b,d,s})
 // for the sites x of one parity; F lexical Field_G (4, V, 18) double.
 hipError_t launch_force_tensor(double* F, const double* Z, const double* Vn, const int* nbr,

@@ -1125,4 +1125,38 @@ hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// rational combination (afopr_Rational-tmpl.h:211-215):
+//   v = a0 b + sum_j c[j] x[j],  j < m;  one read of each field, one write of v
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_lincomb(double* __restrict__ v, const double* __restrict__ b, double a0,
+          const double* const* __restrict__ x, const double* __restrict__ c, int m, size_t n) {
+  PACK_LOOP(double, n) {
+    double acc[2], t[2];
+    Vec<double>::ld(b + ip * 2, acc);
+    acc[0] *= a0;
+    acc[1] *= a0;
+    for (int j = 0; j < m; ++j) {
+      Vec<double>::ld(x[j] + ip * 2, t);
+      const double cj = c[j];
+      acc[0] += cj * t[0];
+      acc[1] += cj * t[1];
+    }
+    Vec<double>::st(v + ip * 2, acc);
+  }
+  TAIL_LOOP(n) {
+    double acc = a0 * b[i];
+    for (int j = 0; j < m; ++j) acc += c[j] * x[j][i];
+    v[i] = acc;
+  }
+}
+
+hipError_t launch_lincomb(void* v, const void* b, double a0, void* const* x, const double* c,
+                          int m, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_lincomb, red_grid(n), dim3(256), 0, st, (double*)v, (const double*)b, a0,
+                     (const double* const*)x, c, m, n);
+  return hipGetLastError();
+}
+
 }  // namespace dwf

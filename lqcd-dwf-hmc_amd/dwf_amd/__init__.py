@@ -385,6 +385,88 @@ class AForce_F_Domainwall_5din_eo:
             self._h = None
 
 
+class Rational:
+    """Coefficients of r(x) = a0 + sum_i bl[i] / (x + cl[i]) with the shift
+    solver's parameters (Math_Rational + the solver block of the YAML)."""
+
+    def __init__(self, a0, bl, cl, maximum_number_of_iteration=10000,
+                 convergence_criterion_squared=1e-24):
+        self.a0 = float(a0)
+        self.bl = np.ascontiguousarray(bl, np.float64)
+        self.cl = np.ascontiguousarray(cl, np.float64)
+        assert self.bl.shape == self.cl.shape and self.bl.size >= 1
+        self.max_iter = int(maximum_number_of_iteration)
+        self.stop_cond = float(convergence_criterion_squared)
+
+    @classmethod
+    def from_file(cls, path, **solver):
+        """A Math_Rational coefficient file (see dwf_amd.lattice.read_rational)."""
+        from .lattice import read_rational
+        d = read_rational(path)
+        return cls(d["a0"], d["res"], d["pole"], **solver)
+
+    def params(self):
+        p = _lib.RationalParams()
+        p.np = self.bl.size
+        p.a0 = self.a0
+        p.bl = _ptr(self.bl)
+        p.cl = _ptr(self.cl)
+        p.max_iter = self.max_iter
+        p.stop_cond = self.stop_cond
+        return p
+
+
+class AFopr_Rational:
+    """AFopr_Rational (src/lib/Fopr/afopr_Rational-tmpl.h:181-219): v = r(A^dag A) b."""
+
+    def __init__(self, fopr, rational):
+        self.fopr = fopr
+        self.rational = rational
+        self.nconv = None
+        self.diff = None
+
+    def mult(self, v, b):
+        nconv, diff = C.c_int(0), np.zeros(1)
+        p = self.rational.params()
+        _lib.check("dwf_rational_mult", _lib.load().dwf_rational_mult(
+            self.fopr._h, C.byref(p), v.ptr, b.ptr, C.byref(nconv), _ptr(diff)))
+        self.nconv, self.diff = nconv.value, float(diff[0])
+
+
+class AForce_F_Rational_Ratio:
+    """AForce_F_Rational_Ratio (src/lib_alt/Force/Fermion/aforce_F_Rational_Ratio-tmpl.h):
+    force of <phi| r1(A^dag A) r2(B^dag B) r1(A^dag A) |phi>, A = fopr1, B = fopr2."""
+
+    def __init__(self, fopr1, fopr2, numerator, denominator):
+        h = C.c_void_p()
+        p1, p2 = numerator.params(), denominator.params()
+        _lib.check("dwf_force_rational_ratio_create", _lib.load().dwf_force_rational_ratio_create(
+            fopr1._h, fopr2._h, C.byref(p1), C.byref(p2), C.byref(h)))
+        self._h = h.value
+        self.fopr1, self.fopr2 = fopr1, fopr2
+        V = int(np.prod(fopr1.lattice_size))
+        self.gauge_shape = (4, V, 18)
+
+    def new_force(self):
+        return DeviceArray(self.gauge_shape)
+
+    def force_udiv(self, F, eta):
+        _lib.check("dwf_force_rational_ratio_udiv",
+                   _lib.load().dwf_force_rational_ratio_udiv(self._h, F.ptr, eta.ptr))
+
+    def force_core(self, F, eta):
+        _lib.check("dwf_force_rational_ratio_core",
+                   _lib.load().dwf_force_rational_ratio_core(self._h, F.ptr, eta.ptr))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.load().dwf_force_rational_ratio_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
 class Director_alt_Smear:
     """Stout smearing levels (APE staples + stout projection) on the GPU."""
 

@@ -44,6 +44,17 @@ class FoprParams(C.Structure):
     ]
 
 
+class RationalParams(C.Structure):
+    _fields_ = [
+        ("np", C.c_int),
+        ("a0", C.c_double),
+        ("bl", _dp),
+        ("cl", _dp),
+        ("max_iter", C.c_int),
+        ("stop_cond", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "dwf_last_error": (C.c_char_p, []),
@@ -99,6 +110,12 @@ _SIGS = {
     "dwf_force_mult_generator": (C.c_int, [_vp, _vp]),
     "dwf_force_core": (C.c_int, [_vp, _vp, _vp]),
     "dwf_force_core1": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dwf_rational_mult": (C.c_int, [_vp, C.POINTER(RationalParams), _vp, _vp, _ip, _dp]),
+    "dwf_force_rational_ratio_create": (C.c_int, [_vp, _vp, C.POINTER(RationalParams),
+                                                  C.POINTER(RationalParams), C.POINTER(_vp)]),
+    "dwf_force_rational_ratio_destroy": (C.c_int, [_vp]),
+    "dwf_force_rational_ratio_udiv": (C.c_int, [_vp, _vp, _vp]),
+    "dwf_force_rational_ratio_core": (C.c_int, [_vp, _vp, _vp]),
     "dwf_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),
     "dwf_device_free": (C.c_int, [_vp]),
     "dwf_memcpy_to_device": (C.c_int, [_vp, _vp, C.c_size_t]),

@@ -178,6 +178,39 @@ class OracleDWF:
         return xs, nconv.value, float(diff[0]), rc
 
 
+def rational_mult(orc, a0, bl, cl, b, niter=10000, stop_cond2=1e-24):
+    """AFopr_Rational::mult (src/lib/Fopr/afopr_Rational-tmpl.h:181-219):
+    returns (v, x) with x_i = (A^dag A + cl_i)^-1 b and v = a0 b + sum_i bl_i x_i."""
+    xs, _, _, _ = orc.shift_cg(b, cl, "DdagD", niter, stop_cond2)
+    v = a0 * b
+    for bi, xi in zip(bl, xs):
+        v = v + bi * xi
+    return v, xs
+
+
+def rational_ratio_force_udiv(orc1, orc2, rat1, rat2, eta, niter=10000, stop_cond2=1e-24):
+    """AForce_F_Rational_Ratio::force_udiv_impl
+    (src/lib_alt/Force/Fermion/aforce_F_Rational_Ratio-tmpl.h:345-447), composed
+    from the oracle's multishift CG, force_udiv and force_udiv1.
+    rat = (a0, bl, cl) of the numerator (orc1 = A) and denominator (orc2 = B)."""
+    a01, bl1, cl1 = rat1
+    a02, bl2, cl2 = rat2
+    v1, eta1 = rational_mult(orc1, a01, bl1, cl1, eta, niter, stop_cond2)
+    v2, eta2 = rational_mult(orc2, a02, bl2, cl2, v1, niter, stop_cond2)
+    zeta1, _, _, _ = orc1.shift_cg(v2, cl1, "DdagD", niter, stop_cond2)
+    F = np.zeros(orc1._gauge_shape())
+    for b, e2 in zip(bl2, eta2):
+        F += b * orc2.force_udiv(e2)
+    for b, e1, z1 in zip(bl1, eta1, zeta1):
+        h1 = orc1.mult("H", e1)
+        h2 = orc1.mult("H", z1)
+        F += b * orc1.force_udiv1("H", h1, z1)
+        F += b * orc1.force_udiv1("H", h2, e1)
+        F += b * orc1.force_udiv1("Hdag", e1, h2)
+        F += b * orc1.force_udiv1("Hdag", z1, h1)
+    return F
+
+
 def richardson(op64, op32, b, mode="DdagD", niter_outer=100, stop_outer2=1e-28,
                niter_inner=10000, stop_inner2=1e-8):
     b = np.ascontiguousarray(b, np.float64)
