@@ -260,6 +260,21 @@ const double* dwf_smear_config_device(dwf_smear sm, int level);
  * smear U, then give the top level to the kernel operator. */
 int dwf_fopr_set_config_smeared(dwf_fopr op, dwf_smear sm, const double* U_host);
 
+/* Stout force recursion (FP64, device Field_G (18, V, 4) arrays).
+ * dwf_smear_force_udiv: Director_alt_Smear::force_udiv of one level
+ *   (src/lib_alt/Smear/director_alt_Smear-tmpl.h:253-270 -> AForceSmear_APE::force_udiv,
+ *   src/lib_alt_QXS/Smear/aforceSmear_APE-tmpl.h:157-206, AProjection_Stout_SU3::
+ *   force_recursive, aprojection_Stout_SU3-tmpl.h:691-895): Sigma = dS/dU_level from
+ *   Sigmap = dS/dU_(level+1); Sigma may alias Sigmap.
+ * dwf_smear_mult_jacobian: AForce_F_Smeared_alt::mult_jacobian
+ *   (src/lib_alt/Force/Fermion/aforce_F_Smeared_alt-tmpl.h:170-191), in place, all levels.
+ * dwf_smear_mult_generator: F <- -2 at(U_0 F) with the thin links U_0.
+ * The force of an operator on smeared links (AForce_F_Smeared_alt::force_core):
+ *   dwf_force_udiv(f, F, eta); dwf_smear_mult_jacobian(sm, F); dwf_smear_mult_generator(sm, F). */
+int dwf_smear_force_udiv(dwf_smear sm, int level, double* Sigma, const double* Sigmap);
+int dwf_smear_mult_jacobian(dwf_smear sm, double* F);
+int dwf_smear_mult_generator(dwf_smear sm, double* F);
+
 /* Average plaquette of a host Field_G (1 for the unit field). */
 int dwf_plaquette(const int lattice_size[4], const double* U_host, double* result);
 

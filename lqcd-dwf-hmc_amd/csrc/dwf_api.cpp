@@ -1545,6 +1545,8 @@ struct dwf_smear_s {
   int nsmear;
   size_t V;
   std::vector<double*> levels;
+  bool have_config = false;
+  double* itheta = nullptr;   // scratch of the force recursion (lazy)
 };
 
 int dwf_smear_create(const int dims[4], double rho, int nsmear, dwf_smear* out) {
@@ -1572,6 +1574,7 @@ int dwf_smear_destroy(dwf_smear sm) {
   if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
   for (auto l : sm->levels)
     if (l) (void)hipFree(l);
+  if (sm->itheta) (void)hipFree(sm->itheta);
   delete sm;
   return DWF_OK;
 }
@@ -1579,6 +1582,7 @@ static int smear_levels(dwf_smear sm) {
   for (int i = 0; i < sm->nsmear; ++i)
     HIPCHK(launch_stout(sm->levels[i + 1], sm->levels[i], sm->N[0], sm->N[1], sm->N[2], sm->N[3],
                         sm->rho, g_ctx.stream));
+  sm->have_config = true;
   return DWF_OK;
 }
 int dwf_smear_set_config_device(dwf_smear sm, const double* U) {
@@ -1610,6 +1614,37 @@ const double* dwf_smear_config_device(dwf_smear sm, int level) {
   if (!sm || level < 0 || level > sm->nsmear) return nullptr;
   return sm->levels[level];
 }
+// Director_alt_Smear::force_udiv for one level (director_alt_Smear-tmpl.h:253-270 ->
+// AForceSmear_APE::force_udiv); Sigma may alias Sigmap (each link is read and
+// written by one thread only)
+int dwf_smear_force_udiv(dwf_smear sm, int level, double* Sigma, const double* Sigmap) {
+  CTX();
+  if (!sm || !Sigma || !Sigmap) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (level < 0 || level >= sm->nsmear) return fail(DWF_ERR_INVALID_ARG, "no such smearing level");
+  if (!sm->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  if (!sm->itheta) HIPCHK(hipMalloc(&sm->itheta, sm->V * 4 * 18 * sizeof(double)));
+  HIPCHK(launch_stout_force(Sigma, sm->itheta, Sigmap, sm->levels[level], sm->N[0], sm->N[1],
+                            sm->N[2], sm->N[3], sm->rho, g_ctx.stream));
+  return DWF_OK;
+}
+// AForce_F_Smeared_alt::mult_jacobian (aforce_F_Smeared_alt-tmpl.h:170-191), in place
+int dwf_smear_mult_jacobian(dwf_smear sm, double* F) {
+  CTX();
+  if (!sm || !F) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  for (int ismear = sm->nsmear - 1; ismear >= 0; --ismear)
+    RC(dwf_smear_force_udiv(sm, ismear, F, F));
+  return DWF_OK;
+}
+// AForce_F::mult_generator with the thin links (m_Ucp of AForce_F_Smeared_alt,
+// aforce_F_Smeared_alt-tmpl.h:72-80; lib_alt_QXS/Force/Fermion/aforce_F.cpp:97-110)
+int dwf_smear_mult_generator(dwf_smear sm, double* F) {
+  CTX();
+  if (!sm || !F) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (!sm->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  HIPCHK(launch_mult_generator_gauge(F, sm->levels[0], sm->V * 4, g_ctx.stream));
+  return DWF_OK;
+}
+
 int dwf_fopr_set_config_smeared(dwf_fopr op, dwf_smear sm, const double* U_host) {
   CTX();
   if (!op || !sm) return fail(DWF_ERR_INVALID_ARG, "null argument");

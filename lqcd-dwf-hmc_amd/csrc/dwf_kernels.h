@@ -106,6 +106,14 @@ hipError_t launch_mult_generator(double* F, const double* links, int Nx, int Ny,
 // Gauge: one stout smearing level, lexical Field_G (4, V, 18) double device.
 hipError_t launch_stout(double* Uout, const double* Uin, int Nx, int Ny, int Nz, int Nt,
                         double rho, hipStream_t st);
+// One level of the stout force recursion: Sigma = dS/dU from Sigmap = dS/dU'
+// (U' = stout(U)); iTheta: scratch gauge field.  Lexical Field_G double device.
+hipError_t launch_stout_force(double* Sigma, double* iTheta, const double* Sigmap,
+                              const double* Uin, int Nx, int Ny, int Nz, int Nt, double rho,
+                              hipStream_t st);
+// F <- -2 at(U F) over nlinks = 4 V links of a lexical gauge field U
+hipError_t launch_mult_generator_gauge(double* F, const double* U, size_t nlinks,
+                                       hipStream_t st);
 // plaquette sum (un-normalised) into result[slot]
 hipError_t launch_plaquette(const double* U, int Nx, int Ny, int Nz, int Nt,
                             const RedScratch& red, int slot, hipStream_t st);

@@ -490,6 +490,18 @@ class Director_alt_Smear:
                    _lib.load().dwf_smear_get_config(self._h, int(level), _ptr(out)))
         return out
 
+    def force_udiv(self, Sigma, Sigmap, level):
+        """Director_alt_Smear::force_udiv of one level (device arrays; may alias)."""
+        _lib.check("dwf_smear_force_udiv", _lib.load().dwf_smear_force_udiv(
+            self._h, int(level), Sigma.ptr, Sigmap.ptr))
+
+    def mult_jacobian(self, F):
+        _lib.check("dwf_smear_mult_jacobian", _lib.load().dwf_smear_mult_jacobian(self._h, F.ptr))
+
+    def mult_generator(self, F):
+        _lib.check("dwf_smear_mult_generator",
+                   _lib.load().dwf_smear_mult_generator(self._h, F.ptr))
+
     def __del__(self):
         if getattr(self, "_h", None):
             try:
@@ -497,6 +509,39 @@ class Director_alt_Smear:
             except Exception:
                 pass
             self._h = None
+
+
+class AForce_F_Smeared_alt:
+    """AForce_F_Smeared_alt (src/lib_alt/Force/Fermion/aforce_F_Smeared_alt-tmpl.h):
+    the force of an operator whose links are the top level of a smearing
+    director: the inner force on the smeared links, mult_jacobian down to the
+    thin links, then mult_generator with the thin links."""
+
+    def __init__(self, force, smear):
+        self.force = force
+        self.smear = smear
+
+    def new_force(self):
+        return self.force.new_force()
+
+    def set_mode(self, mode):
+        self.force.set_mode(mode)
+
+    def force_udiv(self, F, eta):
+        self.force.force_udiv(F, eta)
+        self.smear.mult_jacobian(F)
+
+    def force_udiv1(self, F, zeta, eta):
+        self.force.force_udiv1(F, zeta, eta)
+        self.smear.mult_jacobian(F)
+
+    def force_core(self, F, eta):
+        self.force_udiv(F, eta)
+        self.smear.mult_generator(F)
+
+    def force_core1(self, F, zeta, eta):
+        self.force_udiv1(F, zeta, eta)
+        self.smear.mult_generator(F)
 
 
 def plaquette(lattice_size, U):

@@ -126,6 +126,9 @@ _SIGS = {
     "dwf_smear_set_config_device": (C.c_int, [_vp, _vp]),
     "dwf_smear_get_config": (C.c_int, [_vp, C.c_int, _dp]),
     "dwf_smear_config_device": (_vp, [_vp, C.c_int]),
+    "dwf_smear_force_udiv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "dwf_smear_mult_jacobian": (C.c_int, [_vp, _vp]),
+    "dwf_smear_mult_generator": (C.c_int, [_vp, _vp]),
     "dwf_fopr_set_config_smeared": (C.c_int, [_vp, _vp, _dp]),
     "dwf_plaquette": (C.c_int, [_ip, _dp, _dp]),
 }

@@ -1239,6 +1239,173 @@ void orc_stout_smear(const int dims[4], double rho, double* Uout, const double* 
   }
 }
 
+// ---- stout force recursion --------------------------------------------------
+// AForceSmear_APE::force_udiv for one level (lib_alt_QXS/Smear/aforceSmear_APE-tmpl.h:157-206):
+// given Sigma' = dS/dU' of the smeared links U' = exp(iQ) U, returns Sigma = dS/dU.
+// Per link, AProjection_Stout_SU3::force_recursive (aprojection_Stout_SU3-tmpl.h:691-895),
+// the chain rule of Morningstar and Peardon, Phys. Rev. D 69 (2004) 054501, eqs. (54)-(75):
+//   Gamma = tr(U S' B1) Q + tr(U S' B2) Q^2 + f1 U S' + f2 (Q U S' + U S' Q),
+//   i Lambda = at(i Gamma),  iTheta = i Lambda U,  Xi = S' exp(iQ) + C^+ i Lambda;
+// then the staple terms of force_each (aforceSmear_APE-tmpl.h:280-360).
+
+namespace {
+
+inline M3 add3(const M3& x, const M3& y) {
+  M3 r;
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.a[i][j] = x.a[i][j] + y.a[i][j];
+  return r;
+}
+inline M3 scal3(cd s, const M3& x) {
+  M3 r;
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.a[i][j] = s * x.a[i][j];
+  return r;
+}
+inline cd tr3(const M3& x) { return x.a[0][0] + x.a[1][1] + x.a[2][2]; }
+inline M3 poly3(cd c0, cd c1, cd c2, const M3& Q, const M3& Q2) {
+  M3 r = add3(scal3(c1, Q), scal3(c2, Q2));
+  for (int i = 0; i < 3; ++i) r.a[i][i] += c0;
+  return r;
+}
+
+// xi1(w) = (w cos w - sin w) / w^3, by its Taylor series for small w
+double stout_xi1(double w) {
+  if (w < 0.25) {
+    const double w2 = w * w;
+    return -1.0 / 3.0 + w2 * (1.0 / 30.0 + w2 * (-1.0 / 840.0 + w2 * (1.0 / 45360.0 - w2 / 3991680.0)));
+  }
+  return (w * std::cos(w) - std::sin(w)) / (w * w * w);
+}
+
+// one link: Xi and iTheta from C, U and Sigma'
+void stout_link_force(const M3& C, const M3& U, const M3& Sp, M3& Xi, M3& iTheta) {
+  M3 iQ = at3(mul(C, dag(U)));
+  M3 iQ2 = mul(iQ, iQ), iQ3 = mul(iQ, iQ2);
+  double n2 = 0;
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) n2 += std::norm(iQ.a[i][j]);
+  M3 eiQ = unit3(), iGam = zero3();
+  if (n2 > 1.0e-10) {
+    const M3 Q = scal3(cd(0, -1), iQ);
+    const M3 Q2 = scal3(-1.0, iQ2);
+    double c0 = -(iQ3.a[0][0].imag() + iQ3.a[1][1].imag() + iQ3.a[2][2].imag()) / 3.0;
+    double c1 = -0.5 * (iQ2.a[0][0].real() + iQ2.a[1][1].real() + iQ2.a[2][2].real());
+    double c13r = std::sqrt(c1 / 3.0);
+    double ratio = c0 / (2.0 * c13r * c13r * c13r);
+    if (ratio > 1.0) ratio = 1.0;
+    if (ratio < -1.0) ratio = -1.0;
+    double theta = std::acos(ratio);
+    double u = c13r * std::cos(theta / 3.0), w = std::sqrt(c1) * std::sin(theta / 3.0);
+    double u2 = u * u, w2 = w * w, cw = std::cos(w);
+    double xi0 = (w == 0.0) ? 1.0 : std::sin(w) / w, xi1 = stout_xi1(w);
+    cd emiu(std::cos(u), -std::sin(u)), e2iu(std::cos(2.0 * u), std::sin(2.0 * u));
+    // f_j (eq. 29-30)
+    cd h0 = e2iu * (u2 - w2) + emiu * cd(8.0 * u2 * cw, 2.0 * u * (3.0 * u2 + w2) * xi0);
+    cd h1 = e2iu * (2.0 * u) - emiu * cd(2.0 * u * cw, -(3.0 * u2 - w2) * xi0);
+    cd h2 = e2iu - emiu * cd(cw, 3.0 * u * xi0);
+    double fd = 1.0 / (9.0 * u2 - w2);
+    cd f0 = h0 * fd, f1 = h1 * fd, f2 = h2 * fd;
+    // r_j^(1), r_j^(2) (eqs. 60-65)
+    cd r01 = cd(2.0 * u, 2.0 * (u2 - w2)) * e2iu +
+             emiu * cd(16.0 * u * cw + 2.0 * u * (3.0 * u2 + w2) * xi0,
+                       -8.0 * u2 * cw + 2.0 * (9.0 * u2 + w2) * xi0);
+    cd r11 = cd(2.0, 4.0 * u) * e2iu + emiu * cd(-2.0 * cw + (3.0 * u2 - w2) * xi0,
+                                                2.0 * u * cw + 6.0 * u * xi0);
+    cd r21 = cd(0.0, 2.0) * e2iu + emiu * cd(-3.0 * u * xi0, cw - 3.0 * xi0);
+    cd r02 = cd(-2.0, 0.0) * e2iu + emiu * cd(-8.0 * u2 * xi0, 2.0 * u * (cw + xi0 + 3.0 * u2 * xi1));
+    cd r12 = emiu * cd(2.0 * u * xi0, -cw - xi0 + 3.0 * u2 * xi1);
+    cd r22 = emiu * cd(xi0, -3.0 * u * xi1);
+    // b_ij (eqs. 57-58)
+    double bd = 1.0 / (2.0 * (9.0 * u2 - w2) * (9.0 * u2 - w2));
+    double g1 = 3.0 * u2 - w2, g2 = 30.0 * u2 + 2.0 * w2;
+    cd b10 = (2.0 * u * r01 + g1 * r02 - g2 * f0) * bd;
+    cd b11 = (2.0 * u * r11 + g1 * r12 - g2 * f1) * bd;
+    cd b12 = (2.0 * u * r21 + g1 * r22 - g2 * f2) * bd;
+    cd b20 = (r01 - 3.0 * u * r02 - 24.0 * u * f0) * bd;
+    cd b21 = (r11 - 3.0 * u * r12 - 24.0 * u * f1) * bd;
+    cd b22 = (r21 - 3.0 * u * r22 - 24.0 * u * f2) * bd;
+    M3 B1 = poly3(b10, b11, b12, Q, Q2), B2 = poly3(b20, b21, b22, Q, Q2);
+    M3 US = mul(U, Sp);
+    cd t1 = tr3(mul(US, B1)), t2 = tr3(mul(US, B2));
+    M3 Gam = add3(add3(scal3(t1, Q), scal3(t2, Q2)),
+                  add3(scal3(f1, US), scal3(f2, add3(mul(Q, US), mul(US, Q)))));
+    iGam = scal3(cd(0, 1), Gam);
+    eiQ = poly3(f0, f1, f2, Q, Q2);
+  }
+  const M3 iLam = at3(iGam);
+  iTheta = mul(iLam, U);
+  Xi = add3(mul(Sp, eiQ), mul(dag(C), iLam));
+}
+
+}  // namespace
+
+void orc_stout_force_udiv(const int dims[4], double rho, double* Sigma, const double* Sigmap,
+                          const double* Uin) {
+  GaugeGeom g;
+  for (int mu = 0; mu < 4; ++mu) g.N[mu] = dims[mu];
+  g.V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+  const long V = g.V;
+  auto Ul = [&](long site, int mu) { return load3(Uin + 18 * (site + V * mu)); };
+  std::vector<double> iTh(18L * V * 4);
+  auto Th = [&](long site, int mu) { return load3(iTh.data() + 18 * (site + V * mu)); };
+#pragma omp parallel for schedule(static)
+  for (long site = 0; site < V; ++site) {
+    for (int mu = 0; mu < 4; ++mu) {
+      M3 C = zero3();
+      for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        long xpmu = g.shift(site, mu, +1), xpnu = g.shift(site, nu, +1);
+        long xmnu = g.shift(site, nu, -1), xmnupmu = g.shift(xmnu, mu, +1);
+        M3 up = mul(mul(Ul(site, nu), Ul(xpnu, mu)), dag(Ul(xpmu, nu)));
+        M3 lo = mul(mul(dag(Ul(xmnu, nu)), Ul(xmnu, mu)), Ul(xmnupmu, nu));
+        C = add3(C, scal3(rho, add3(up, lo)));
+      }
+      M3 Xi, iTheta;
+      stout_link_force(C, Ul(site, mu), load3(Sigmap + 18 * (site + V * mu)), Xi, iTheta);
+      store3(Sigma + 18 * (site + V * mu), Xi);
+      store3(iTh.data() + 18 * (site + V * mu), iTheta);
+    }
+  }
+  // staple terms: Sigma_mu(x) += rho sum_{nu != mu} (six products of U and iTheta)
+#pragma omp parallel for schedule(static)
+  for (long site = 0; site < V; ++site) {
+    for (int mu = 0; mu < 4; ++mu) {
+      M3 acc = zero3();
+      for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        long xpmu = g.shift(site, mu, +1), xpnu = g.shift(site, nu, +1);
+        long xmnu = g.shift(site, nu, -1), xmnupmu = g.shift(xmnu, mu, +1);
+        M3 t = zero3();
+        // V_nu(x+mu) V_mu(x+nu)^+ iTheta_nu(x)^+
+        t = add3(t, mul(mul(Ul(xpmu, nu), dag(Ul(xpnu, mu))), dag(Th(site, nu))));
+        // V_nu(x-nu+mu)^+ iTheta_mu(x-nu)^+ V_nu(x-nu)
+        t = add3(t, mul(mul(dag(Ul(xmnupmu, nu)), dag(Th(xmnu, mu))), Ul(xmnu, nu)));
+        // V_nu(x-nu+mu)^+ V_mu(x-nu)^+ iTheta_nu(x-nu)
+        t = add3(t, mul(mul(dag(Ul(xmnupmu, nu)), dag(Ul(xmnu, mu))), Th(xmnu, nu)));
+        // iTheta_nu(x+mu) V_mu(x+nu)^+ V_nu(x)^+
+        t = add3(t, mul(mul(Th(xpmu, nu), dag(Ul(xpnu, mu))), dag(Ul(site, nu))));
+        // iTheta_nu(x-nu+mu)^+ V_mu(x-nu)^+ V_nu(x-nu)
+        t = add3(t, mul(mul(dag(Th(xmnupmu, nu)), dag(Ul(xmnu, mu))), Ul(xmnu, nu)));
+        // V_nu(x+mu) iTheta_mu(x+nu)^+ V_nu(x)^+
+        t = add3(t, mul(mul(Ul(xpmu, nu), dag(Th(xpnu, mu))), dag(Ul(site, nu))));
+        acc = add3(acc, scal3(rho, t));
+      }
+      double* out = Sigma + 18 * (site + V * mu);
+      store3(out, add3(load3(out), acc));
+    }
+  }
+}
+
+// mult_generator with a given gauge field: F <- -2 at(U F)
+void orc_mult_generator_gauge(const int dims[4], double* F, const double* U) {
+  const long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+#pragma omp parallel for schedule(static)
+  for (long site = 0; site < V; ++site)
+    for (int mu = 0; mu < 4; ++mu) {
+      double* f = F + 18 * (site + V * mu);
+      M3 A = at3(mul(load3(U + 18 * (site + V * mu)), load3(f)));
+      store3(f, scal3(-2.0, A));
+    }
+}
+
 double orc_plaquette(const int dims[4], const double* U) {
   GaugeGeom g;
   for (int mu = 0; mu < 4; ++mu) g.N[mu] = dims[mu];

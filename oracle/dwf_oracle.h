@@ -88,6 +88,12 @@ double orc_dot(const double* x, const double* y, long n);
 void orc_stout_smear(const int dims[4], double rho, double* Uout,
                      const double* Uin);
 /* Average plaquette, normalised to 1 for the unit field. */
+/* AForceSmear_APE::force_udiv of one stout level: Sigma = dS/dU from
+ * Sigmap = dS/dU' with U' the stout smearing of Uin. */
+void orc_stout_force_udiv(const int dims[4], double rho, double* Sigma,
+                          const double* Sigmap, const double* Uin);
+/* F <- -2 at(U F) with the gauge field U */
+void orc_mult_generator_gauge(const int dims[4], double* F, const double* U);
 double orc_plaquette(const int dims[4], const double* U);
 /* exp(iQ) of one anti-hermitian traceless iQ (18 reals) by Cayley-Hamilton,
  * and by a 32-term Taylor series (the brute-force check). */

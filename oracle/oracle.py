@@ -55,6 +55,8 @@ def lib():
         L.orc_force_udiv.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_mult_generator.argtypes = [C.c_void_p, _dp]
         L.orc_stout_smear.argtypes = [C.POINTER(C.c_int), C.c_double, _dp, _dp]
+        L.orc_stout_force_udiv.argtypes = [C.POINTER(C.c_int), C.c_double, _dp, _dp, _dp]
+        L.orc_mult_generator_gauge.argtypes = [C.POINTER(C.c_int), _dp, _dp]
         L.orc_plaquette.argtypes = [C.POINTER(C.c_int), _dp]
         L.orc_plaquette.restype = C.c_double
         L.orc_exp_iQ.argtypes = [_dp, _dp]
@@ -229,6 +231,31 @@ def stout_smear(dims, rho, U):
     out = np.empty_like(U)
     lib().orc_stout_smear((C.c_int * 4)(*dims), float(rho), _ptr(out), _ptr(U))
     return out
+
+
+def stout_force_udiv(dims, rho, Sigmap, U):
+    """AForceSmear_APE::force_udiv of one stout level
+    (lib_alt_QXS/Smear/aforceSmear_APE-tmpl.h:157-206): dS/dU from dS/dU'."""
+    Sigmap = np.ascontiguousarray(Sigmap, np.float64)
+    U = np.ascontiguousarray(U, np.float64)
+    out = np.empty_like(U)
+    lib().orc_stout_force_udiv((C.c_int * 4)(*dims), float(rho), _ptr(out), _ptr(Sigmap), _ptr(U))
+    return out
+
+
+def stout_mult_jacobian(dims, rho, levels, F):
+    """AForce_F_Smeared_alt::mult_jacobian (aforce_F_Smeared_alt-tmpl.h:170-191):
+    levels[i] is the input of smearing level i (levels[0] = the thin links)."""
+    for ismear in range(len(levels) - 1, -1, -1):
+        F = stout_force_udiv(dims, rho, F, levels[ismear])
+    return F
+
+
+def mult_generator_gauge(dims, F, U):
+    F = np.ascontiguousarray(F, np.float64).copy()
+    U = np.ascontiguousarray(U, np.float64)
+    lib().orc_mult_generator_gauge((C.c_int * 4)(*dims), _ptr(F), _ptr(U))
+    return F
 
 
 def plaquette(dims, U):

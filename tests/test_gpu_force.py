@@ -92,3 +92,52 @@ def test_rational_ratio_force_matches_oracle(dims, Ls):
     assert g1.get_mode() == "H"
     force.force_core(F, fphi)
     assert rel(F.to_host(), o1.mult_generator(Fu)) < 1e-9
+
+
+@pytest.mark.parametrize("dims,rho,nsmear", [((4, 2, 4, 2), 0.12, 2), ((8, 8, 8, 8), 0.1, 2)])
+def test_stout_force_matches_oracle(dims, rho, nsmear):
+    from oracle import stout_smear, stout_force_udiv, stout_mult_jacobian, mult_generator_gauge
+    print("case", dims, rho, nsmear, flush=True)
+    U = lattice.random_gauge(dims, 0.4, 61)
+    sm = dwf_amd.Director_alt_Smear(dims, rho, nsmear)
+    sm.set_config(U)
+    lv = [U]
+    for _ in range(nsmear):
+        lv.append(stout_smear(dims, rho, lv[-1]))
+    rng = np.random.default_rng(62)
+    S = rng.standard_normal(U.shape)
+    dS = dwf_amd.DeviceArray(U.shape)
+    dS.from_host(S)
+    out = dwf_amd.DeviceArray(U.shape)
+    # one level, out of place
+    sm.force_udiv(out, dS, 1)
+    assert rel(out.to_host(), stout_force_udiv(dims, rho, S, lv[1])) < 1e-12
+    # the whole jacobian in place, then the generator with the thin links
+    want = stout_mult_jacobian(dims, rho, lv[:-1], S)
+    sm.mult_jacobian(dS)
+    assert rel(dS.to_host(), want) < 1e-12
+    sm.mult_generator(dS)
+    assert rel(dS.to_host(), mult_generator_gauge(dims, want, U)) < 1e-12
+
+
+def test_smeared_fermion_force_matches_oracle():
+    from oracle import stout_smear, stout_mult_jacobian, mult_generator_gauge
+    dims, Ls, rho, nsmear = (4, 4, 4, 4), 6, 0.1, 2
+    U = lattice.random_gauge(dims, 0.4, 63)
+    eta = lattice.random_spinor5(dims, Ls, 64)
+    prm = (0.05, 1.6, 1.5, 0.5)
+    lv = [U]
+    for _ in range(nsmear):
+        lv.append(stout_smear(dims, rho, lv[-1]))
+    orc = OracleDWF(dims, Ls, *prm)
+    orc.set_config(lv[-1])
+    want = mult_generator_gauge(dims, stout_mult_jacobian(dims, rho, lv[:-1], orc.force_udiv(eta)), U)
+    gpu = dwf_amd.AFopr_Domainwall_5din_eo(dims, Ls, *prm)
+    sm = dwf_amd.Director_alt_Smear(dims, rho, nsmear)
+    gpu.set_config_smeared(sm, U)
+    fs = dwf_amd.AForce_F_Smeared_alt(dwf_amd.AForce_F_Domainwall_5din_eo(gpu), sm)
+    fe = gpu.new_field()
+    gpu.convert(fe, eta)
+    F = fs.new_force()
+    fs.force_core(F, fe)
+    assert rel(F.to_host(), want) < 1e-12
