@@ -168,6 +168,51 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond,
                              const void* b, int* nconv, double* diff);
 
 /* ------------------------------------------------------------------------
+ * Learned polynomial preconditioner (no counterpart in the reference; it is a
+ * new solver component for the same operator).
+ *   K = sum_{k=0}^{L} W_k H^k,  H = 1 - D,  D the even-odd Schur operator of op,
+ * W_k per-chirality Ls x Ls real matrices (weights), 1 <= L <= 4, Ls <= 16.
+ * K approximates D^-1 and M = K K^+ approximates (D^+ D)^-1 (symmetric by
+ * construction).  dwf_lpc_train fits the weights on the current gauge field
+ * by least squares, min sum_j |K D v_j - v_j|^2 over nvec random fields v_j
+ * (uniform in [-1, 1), counter hash of seed), with ridge * mean(diag) added to
+ * the normal matrix; loss = the relative residual of the fit.  Training runs
+ * with opt (any precision, same lattice and parameters); the preconditioner
+ * is applied in op's precision (FP16/BF16 storage is meant for it).
+ * Weights layout: [k][chi][s_out][s_in], chi = 0 for P+, 1 for P-; a new
+ * preconditioner starts from W_k = 1 (the truncated Neumann series of D^-1).
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_lpc_s* dwf_lpc;
+
+int dwf_lpc_create(dwf_fopr op, int degree, dwf_lpc* out);
+int dwf_lpc_destroy(dwf_lpc p);
+int dwf_lpc_degree(dwf_lpc p);
+int dwf_lpc_set_weights(dwf_lpc p, const double* W);
+int dwf_lpc_get_weights(dwf_lpc p, double* W);
+int dwf_lpc_train(dwf_lpc p, dwf_fopr opt, int nvec, unsigned long long seed, double ridge,
+                  double* loss);
+/* which: "K", "Kdag" or "M" (= K K^+); v != w, fields of p's operator */
+int dwf_lpc_apply(dwf_lpc p, const char* which, void* v, const void* w);
+/* chiral Gram matrices of nfields <= 6 fields of op (all pairs i <= j, row-major
+ * over i): out[p][chi][s][t] = sum_{site, k<12} p_chi(f_i)(s) p_chi(f_j)(t),
+ * p_+ = f_k + f_{k+12}, p_- = f_k - f_{k+12} (host array of nfields(nfields+1)/2 * 2 Ls^2) */
+int dwf_lpc_chiral_gram(dwf_fopr op, int nfields, const void* const* fields, double* out_host);
+/* x_i = uniform [-1, 1) from the counter hash of (seed, i), i the device index */
+int dwf_field_fill_uniform(dwf_fopr op, void* x, unsigned long long seed);
+
+/* Flexible CG on op (mode DdagD) preconditioned by M = K K^+, zero initial
+ * guess; stop when |r|^2 / |b|^2 < stop_cond.  The preconditioner may use a
+ * lower precision than op. */
+int dwf_solver_create_fcg(dwf_fopr op, dwf_lpc prec, int max_iter, double stop_cond,
+                          dwf_solver* out);
+/* ASolver_Richardson (FP64 op_d) whose inner solver is the flexible CG on op_f
+ * with the preconditioner: the mixed-precision solve of Fprop_alt_Standard_eo_Richardson
+ * with the inner CG replaced. */
+int dwf_solver_create_richardson_lpc(dwf_fopr op_d, dwf_fopr op_f, dwf_lpc prec, int max_iter,
+                                     double stop_cond, int max_iter_inner,
+                                     double stop_cond_inner, dwf_solver* out);
+
+/* ------------------------------------------------------------------------
  * DWF fermion force, FP64: AForce_F_Domainwall_5din_eo
  * (src/lib_alt_QXS/Force/Fermion/aforce_F_Domainwall_5din_eo-tmpl.h:299-451,
  *  lexical part aforce_F_Domainwall_5din-tmpl.h:338-933, tensorProd_5din

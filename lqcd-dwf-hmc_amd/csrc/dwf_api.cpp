@@ -219,6 +219,8 @@ enum MatId {
   M_DEE, M_DEET, M_DI,
   M_G5, M_G5R,
   M_M5F,        // -1/2 Mp: M5 weight of the force (aforce_F_Domainwall_5din-tmpl.h:355-365)
+  M_HE2,        // -M_E2: second hop of H = 1 - D
+  M_HG2,        // -M_G2: second hop of H^+ = 1 - D^+
   M_COUNT
 };
 
@@ -242,6 +244,7 @@ struct dwf_fopr_s {
   size_t stage_bytes = 0;
   void* work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool have_config = false;
+  std::vector<ChiMat> hmats;   // host copies of the matrices (long double)
 };
 
 static size_t cbytes(int prec) { return prec == P64 ? 8 : 4; }
@@ -250,25 +253,33 @@ static const void* mat(const dwf_fopr_s* op, int id) {
   return (const char*)op->mats + (size_t)id * 2 * op->Ls * op->Ls * cbytes(op->prec);
 }
 
-static int upload_mats(dwf_fopr_s* op, const std::vector<ChiMat>& ms) {
-  const int L = op->Ls;
+// ChiMats to the device in the compute type of prec, each scaled by 1/2 (the
+// chiral recombination factor of the kernel epilogue)
+static int upload_chimats(int prec, void* dst, const std::vector<ChiMat>& ms) {
+  if (ms.empty()) return DWF_OK;
+  const int L = ms[0].Ls;
   const size_t per = 2 * (size_t)L * L;
-  if (op->prec == P64) {
-    std::vector<double> h(per * M_COUNT);
-    for (int id = 0; id < M_COUNT; ++id)
+  if (prec == P64) {
+    std::vector<double> h(per * ms.size());
+    for (size_t id = 0; id < ms.size(); ++id)
       for (int c = 0; c < 2; ++c)
         for (int i = 0; i < L * L; ++i)
           h[id * per + c * L * L + i] = (double)(0.5L * ms[id].m[c][i]);
-    HIPCHK(hipMemcpy(op->mats, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   } else {
-    std::vector<float> h(per * M_COUNT);
-    for (int id = 0; id < M_COUNT; ++id)
+    std::vector<float> h(per * ms.size());
+    for (size_t id = 0; id < ms.size(); ++id)
       for (int c = 0; c < 2; ++c)
         for (int i = 0; i < L * L; ++i)
           h[id * per + c * L * L + i] = (float)(0.5L * ms[id].m[c][i]);
-    HIPCHK(hipMemcpy(op->mats, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   return DWF_OK;
+}
+
+static int upload_mats(dwf_fopr_s* op, const std::vector<ChiMat>& ms) {
+  op->hmats = ms;
+  return upload_chimats(op->prec, op->mats, ms);
 }
 
 // Builds the per-chirality matrices of the operator.  The 5th-dimension
@@ -350,6 +361,8 @@ static int build_mats(dwf_fopr_s* op) {
   ms[M_G5] = G5;
   ms[M_G5R] = G5R;
   ms[M_M5F] = cm_scale(Mp, -0.5L);
+  ms[M_HE2] = cm_scale(ms[M_E2], -1.0L);
+  ms[M_HG2] = cm_scale(ms[M_G2], -1.0L);
   return upload_mats(op, ms);
 }
 
@@ -816,8 +829,175 @@ int dwf_field_convert_precision(dwf_fopr op_to, void* v, dwf_fopr op_from, const
 }
 
 // ---- solvers ----------------------------------------------------------------
+// ---- learned polynomial preconditioner ---------------------------------------
+// K = sum_{k=0}^{L} W_k H^k with H = 1 - D (D the even-odd Schur operator) and
+// W_k per-chirality Ls x Ls real matrices fitted by least squares so that
+// K D v ~ v for random v; K approximates D^-1, M = K K^+ approximates
+// (D^+ D)^-1 and is symmetric positive semi-definite by construction.
+//   K w:    acc = W_0 w, z = w; for k = 1..L: z = H z, acc += W_k z.
+//   K^+ w:  t = W_L^T w; for k = L-1..0: t = W_k^T w + H^+ t.
+// Every step is two fused hop kernels: the W_k terms ride in their epilogues
+// (out2 accumulation for K, the chiral-matrix X term for K^+).  With Mp not
+// spin-scalar the K recursion runs on Mp z, so that Mp leaves the hop
+// prologue: the epilogue matrices are Mp (-E2) and W_k Mp^-1.
+
+struct dwf_lpc_s {
+  dwf_fopr op = nullptr;
+  int L = 0;
+  std::vector<ChiMat> W;
+  ChiMat P, Pinv;          // Mp and its inverse (identity when Mp is spin-scalar)
+  void* mats = nullptr;    // 2L + 3 matrices in op's compute type
+  void* f[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* gpart = nullptr;
+  double* gout = nullptr;
+  double loss = -1.0;
+};
+
+// device matrix slots: 0 Q = P(-E2); 1 W_0; 1+k A_k = W_k P^-1 (k = 1..L);
+// L+2 Di^T W_L^T; L+3+k W_k^T (k = 0..L-1)
+static const void* lpc_mat(const dwf_lpc_s* p, int slot) {
+  return (const char*)p->mats + (size_t)slot * 2 * p->op->Ls * p->op->Ls * cbytes(p->op->prec);
+}
+
+static int lpc_upload(dwf_lpc_s* p) {
+  const dwf_fopr op = p->op;
+  const int L = p->L;
+  std::vector<ChiMat> ms;
+  ms.push_back(cm_mul(p->P, op->hmats[M_HE2]));
+  ms.push_back(p->W[0]);
+  for (int k = 1; k <= L; ++k) ms.push_back(cm_mul(p->W[k], p->Pinv));
+  ms.push_back(cm_mul(op->hmats[M_DIT], cm_transpose(p->W[L])));
+  for (int k = 0; k < L; ++k) ms.push_back(cm_transpose(p->W[k]));
+  return upload_chimats(op->prec, p->mats, ms);
+}
+
+static int lpc_field(dwf_lpc_s* p, int i, void** out) {
+  if (!p->f[i]) HIPCHK(hipMalloc(&p->f[i], p->op->nelem * prec_bytes(p->op->prec)));
+  *out = p->f[i];
+  return DWF_OK;
+}
+
+static int site_m(dwf_fopr op, void* out, const void* in, const void* E, void* out2 = nullptr,
+                  const void* F = nullptr) {
+  Epi e;
+  e.E = E;
+  e.out2 = out2;
+  e.F = F;
+  HIPCHK(launch_dwf(op->prec, K_SITE, out, in, op->links, op->nbr, 0, op->Vh, op->Ls, e,
+                    g_ctx.stream));
+  return DWF_OK;
+}
+
+// v = K w  (v != w)
+static int lpc_apply_K(dwf_lpc_s* p, void* v, const void* w) {
+  dwf_fopr op = p->op;
+  const int L = p->L;
+  if (L == 0) return site_m(op, v, w, lpc_mat(p, 1));
+  void *zh, *t1, *tmp;
+  RC(lpc_field(p, 0, &zh));
+  RC(lpc_field(p, 1, &t1));
+  RC(lpc_field(p, 2, &tmp));
+  const void* zin = w;
+  if (!op->mp_scalar) {
+    RC(site_m(op, zh, w, mat(op, M_PRE), v, lpc_mat(p, 1)));   // zh = Mp w, v = W_0 w
+    zin = zh;
+  } else {
+    RC(site_m(op, v, w, lpc_mat(p, 1)));
+  }
+  for (int k = 1; k <= L; ++k) {
+    RC(run_hop(op, false, 1, t1, zin, M_E1));
+    Epi e;
+    e.E = lpc_mat(p, 0);
+    e.out2 = v;
+    e.F = lpc_mat(p, 1 + k);
+    e.X2 = v;
+    HIPCHK(launch_dwf(op->prec, K_HOP, k < L ? zh : tmp, t1, op->links, op->nbr, 0, op->Vh,
+                      op->Ls, e, g_ctx.stream));
+    zin = zh;
+  }
+  return DWF_OK;
+}
+
+// v = K^+ w  (v != w)
+static int lpc_apply_Kdag(dwf_lpc_s* p, void* v, const void* w) {
+  dwf_fopr op = p->op;
+  const int L = p->L;
+  if (L == 0) return site_m(op, v, w, lpc_mat(p, L + 3));
+  void *th, *t1, *tmp;
+  RC(lpc_field(p, 0, &th));
+  RC(lpc_field(p, 1, &t1));
+  RC(lpc_field(p, 2, &tmp));
+  RC(site_m(op, th, w, lpc_mat(p, L + 2)));   // th = Di^T W_L^T w
+  for (int k = L - 1; k >= 0; --k) {
+    RC(run_hop(op, true, 1, t1, th, M_G1));
+    Epi e;
+    e.E = mat(op, M_HG2);
+    e.X = w;
+    e.G = lpc_mat(p, L + 3 + k);
+    if (k > 0) {
+      e.out2 = th;
+      e.F = mat(op, M_DIT);
+    }
+    HIPCHK(launch_dwf(op->prec, K_HOPDAG, k > 0 ? tmp : v, t1, op->links, op->nbr, 0, op->Vh,
+                      op->Ls, e, g_ctx.stream));
+  }
+  return DWF_OK;
+}
+
+// v = K K^+ w  (v != w)
+static int lpc_apply_M(dwf_lpc_s* p, void* v, const void* w) {
+  void* t;
+  RC(lpc_field(p, 3, &t));
+  RC(lpc_apply_Kdag(p, t, w));
+  return lpc_apply_K(p, v, t);
+}
+
+// H z = (1 - D) z = -E2 Hop_eo E1 Hop_oe (Mp) z
+static int op_Hschur(dwf_fopr op, void* v, const void* w) {
+  void *t0, *t1;
+  RC(get_work(op, 0, &t0));
+  RC(get_work(op, 1, &t1));
+  const void* src = w;
+  if (!op->mp_scalar) {
+    RC(run_site(op, t0, w, M_PRE));
+    src = t0;
+  }
+  RC(run_hop(op, false, 1, t1, src, M_E1));
+  RC(run_hop(op, false, 0, v, t1, M_HE2));
+  return DWF_OK;
+}
+
+// Cholesky solve of the symmetric positive definite n x n system A X = B (m columns)
+static bool chol_solve(std::vector<long double>& A, int n, std::vector<long double>& B, int m) {
+  for (int j = 0; j < n; ++j) {
+    long double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0)) return false;
+    d = std::sqrt(d);
+    A[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      long double v = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = v / d;
+    }
+  }
+  for (int c = 0; c < m; ++c) {
+    for (int i = 0; i < n; ++i) {
+      long double v = B[(size_t)i * m + c];
+      for (int k = 0; k < i; ++k) v -= A[(size_t)i * n + k] * B[(size_t)k * m + c];
+      B[(size_t)i * m + c] = v / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      long double v = B[(size_t)i * m + c];
+      for (int k = i + 1; k < n; ++k) v -= A[(size_t)k * n + i] * B[(size_t)k * m + c];
+      B[(size_t)i * m + c] = v / A[(size_t)i * n + i];
+    }
+  }
+  return true;
+}
+
 struct dwf_solver_s {
-  int kind = 0;  // 0 CG, 1 Richardson(mixed)
+  int kind = 0;  // 0 CG, 1 Richardson(mixed), 2 flexible CG preconditioned by M = K K^+
   dwf_fopr op = nullptr;
   dwf_fopr op_f = nullptr;
   int niter = 0;
@@ -825,10 +1005,11 @@ struct dwf_solver_s {
   int guess = DWF_GUESS_RHS;
   int niter_inner = 0;
   double stop_inner = 0;
-  void* f[4] = {nullptr, nullptr, nullptr, nullptr};   // work in op precision
-  void* g[2] = {nullptr, nullptr};                     // work in op_f precision
+  void* f[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // work in op precision
+  void* g[2] = {nullptr, nullptr};   // work in op_f (kind 1) or the preconditioner's precision
   dwf_solver_s* inner = nullptr;
   int inner_iters = 0;
+  dwf_lpc_s* lpc = nullptr;
 };
 
 static int solver_field(dwf_fopr op, void** slot) {
@@ -898,6 +1079,81 @@ static int cg_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out, do
   return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "ASolver_CG: not converged") : DWF_OK;
 }
 
+// Flexible preconditioned CG on A = op (mode DdagD) with M = K K^+, zero
+// initial guess:  beta = <z', r' - r> / <z, r> = -alpha <z', A p> / <z, r>.
+// The preconditioner may run in another precision than op (conversions at
+// its input and output).
+static int lpc_apply_conv(dwf_solver_s* s, void* z, const void* r) {
+  dwf_lpc_s* P = s->lpc;
+  dwf_fopr op = s->op;
+  if (P->op->prec == op->prec) return lpc_apply_M(P, z, r);
+  RC(solver_field(P->op, &s->g[0]));
+  RC(solver_field(P->op, &s->g[1]));
+  HIPCHK(launch_convert(P->op->prec, s->g[0], op->prec, r, op->nelem, g_ctx.stream));
+  RC(lpc_apply_M(P, s->g[1], s->g[0]));
+  HIPCHK(launch_convert(op->prec, z, P->op->prec, s->g[1], op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+
+static int fcg_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out, double* diff_out) {
+  dwf_fopr op = s->op;
+  for (int i = 0; i < 5; ++i) RC(solver_field(op, &s->f[i]));
+  void *x = s->f[0], *r = s->f[1], *p = s->f[2], *q = s->f[3], *z = s->f[4];
+  const int pr = op->prec;
+  const size_t n = op->nelem;
+  hipStream_t st = g_ctx.stream;
+  double sr;
+  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &sr));
+  if (!(sr > 0)) {
+    HIPCHK(launch_set_zero(pr, xq, n, st));
+    *nconv_out = 0;
+    *diff_out = 0;
+    s->inner_iters = 0;
+    return DWF_OK;
+  }
+  const double snorm = 1.0 / sr;
+  HIPCHK(launch_copy(pr, r, b, n, st));
+  HIPCHK(launch_set_zero(pr, x, n, st));
+  RC(lpc_apply_conv(s, z, r));
+  double rz;
+  HIPCHK(launch_dot(pr, r, z, n, g_ctx.red, 0, st));
+  RC(fetch(1, &rz));
+  HIPCHK(launch_copy(pr, p, z, n, st));
+  int nconv = -1;
+  for (int iter = 0; iter < s->niter; ++iter) {
+    RC(mult_impl(op, q, p, op->mode, false));
+    double pq;
+    HIPCHK(launch_dot(pr, p, q, n, g_ctx.red, 0, st));
+    RC(fetch(1, &pq));
+    const double alpha = rz / pq;
+    double rr;
+    HIPCHK(launch_cg_xr(pr, x, r, p, q, alpha, n, g_ctx.red, 0, st));
+    RC(fetch(1, &rr));
+    if (rr * snorm < s->stop) {
+      nconv = iter + 1;
+      break;
+    }
+    RC(lpc_apply_conv(s, z, r));
+    double d[2];
+    HIPCHK(launch_dot(pr, r, z, n, g_ctx.red, 0, st));
+    HIPCHK(launch_dot(pr, z, q, n, g_ctx.red, 1, st));
+    RC(fetch(2, d));
+    const double beta = -alpha * d[1] / rz;
+    rz = d[0];
+    HIPCHK(launch_aypx(pr, beta, p, z, n, st));
+  }
+  HIPCHK(launch_copy(pr, xq, x, n, st));
+  RC(mult_impl(op, q, xq, op->mode, false));
+  double diff2;
+  HIPCHK(launch_sub_norm2(pr, q, q, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &diff2));
+  *nconv_out = nconv;
+  *diff_out = std::sqrt(diff2 * snorm);
+  s->inner_iters = nconv > 0 ? nconv : s->niter;
+  return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "flexible CG: not converged") : DWF_OK;
+}
+
 // ASolver_Richardson::solve with APrecond_Mixedprec
 // (asolver_Richardson-tmpl.h:131-213, aprecond_Mixedprec-tmpl.h:55-108)
 static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
@@ -938,7 +1194,8 @@ static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv
     HIPCHK(launch_convert(opf->prec, rf, pr, r, n, st));
     int inc = 0;
     double idiff = 0;
-    int rc = cg_solve(s->inner, sf, rf, &inc, &idiff);
+    int rc = s->inner->kind == 2 ? fcg_solve(s->inner, sf, rf, &inc, &idiff)
+                                 : cg_solve(s->inner, sf, rf, &inc, &idiff);
     if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
     s->inner_iters += s->inner->inner_iters;
     HIPCHK(launch_convert(pr, sv, opf->prec, sf, n, st));
@@ -995,6 +1252,36 @@ int dwf_solver_create_richardson(dwf_fopr op_d, dwf_fopr op_f, int max_iter, dou
   return DWF_OK;
 }
 
+int dwf_solver_create_fcg(dwf_fopr op, dwf_lpc prec, int max_iter, double stop_cond,
+                          dwf_solver* out) {
+  CTX();
+  if (!op || !prec || !out || max_iter < 0) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (op->nelem != prec->op->nelem) return fail(DWF_ERR_INVALID_ARG, "operators differ in size");
+  dwf_solver_s* s = new dwf_solver_s;
+  s->kind = 2;
+  s->op = op;
+  s->niter = max_iter;
+  s->stop = stop_cond;
+  s->guess = DWF_GUESS_ZERO;
+  s->lpc = prec;
+  *out = s;
+  return DWF_OK;
+}
+
+int dwf_solver_create_richardson_lpc(dwf_fopr op_d, dwf_fopr op_f, dwf_lpc prec, int max_iter,
+                                     double stop_cond, int max_iter_inner,
+                                     double stop_cond_inner, dwf_solver* out) {
+  CTX();
+  if (!op_d || !op_f || !prec || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (op_d->nelem != op_f->nelem || op_f->nelem != prec->op->nelem)
+    return fail(DWF_ERR_INVALID_ARG, "operators differ in size");
+  RC(dwf_solver_create_richardson(op_d, op_f, max_iter, stop_cond, max_iter_inner,
+                                  stop_cond_inner, out));
+  (*out)->inner->kind = 2;
+  (*out)->inner->lpc = prec;
+  return DWF_OK;
+}
+
 int dwf_solver_solve(dwf_solver s, void* x, const void* b, int* nconv, double* diff) {
   CTX();
   if (!s || !x || !b || !nconv || !diff) return fail(DWF_ERR_INVALID_ARG, "null argument");
@@ -1003,6 +1290,11 @@ int dwf_solver_solve(dwf_solver s, void* x, const void* b, int* nconv, double* d
   if (s->kind == 1) {
     if (s->op_f->mode != s->op->mode) s->op_f->mode = s->op->mode;
     return richardson_solve(s, x, b, nconv, diff);
+  }
+  if (s->kind == 2) {
+    if (s->op->mode != "DdagD")
+      return fail(DWF_ERR_UNSUPPORTED, "the learned preconditioner solves mode DdagD");
+    return fcg_solve(s, x, b, nconv, diff);
   }
   return cg_solve(s, x, b, nconv, diff);
 }
@@ -1018,6 +1310,221 @@ int dwf_solver_destroy(dwf_solver s) {
     if (g) (void)hipFree(g);
   if (s->inner) dwf_solver_destroy(s->inner);
   delete s;
+  return DWF_OK;
+}
+
+int dwf_lpc_create(dwf_fopr op, int degree, dwf_lpc* out) {
+  CTX();
+  if (!op || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (degree < 1 || degree > 4) return fail(DWF_ERR_INVALID_ARG, "degree must be 1..4");
+  if (op->Ls > 16) return fail(DWF_ERR_UNSUPPORTED, "the learned preconditioner needs Ls <= 16");
+  dwf_lpc_s* p = new dwf_lpc_s;
+  p->op = op;
+  p->L = degree;
+  const int Ls = op->Ls;
+  // start from the truncated Neumann series: W_k = 1
+  p->W.assign(degree + 1, cm_identity(Ls));
+  if (op->mp_scalar) {
+    p->P = cm_identity(Ls);
+    p->Pinv = cm_identity(Ls);
+  } else {
+    p->P = op->hmats[M_PRE];
+    if (!cm_inverse(p->P, p->Pinv)) {
+      delete p;
+      return fail(DWF_ERR_UNSUPPORTED, "Mp is singular");
+    }
+  }
+  const size_t nm = 2 * (size_t)degree + 3;
+  int rc = DWF_OK;
+  if (hipMalloc(&p->mats, nm * 2 * Ls * Ls * cbytes(op->prec)) != hipSuccess)
+    rc = fail(DWF_ERR_HIP, "device allocation failed");
+  if (rc == DWF_OK) rc = lpc_upload(p);
+  if (rc != DWF_OK) {
+    if (p->mats) (void)hipFree(p->mats);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return DWF_OK;
+}
+
+int dwf_lpc_destroy(dwf_lpc p) {
+  if (!p) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  for (auto& f : p->f)
+    if (f) (void)hipFree(f);
+  if (p->mats) (void)hipFree(p->mats);
+  if (p->gpart) (void)hipFree(p->gpart);
+  if (p->gout) (void)hipFree(p->gout);
+  delete p;
+  return DWF_OK;
+}
+
+int dwf_lpc_degree(dwf_lpc p) { return p ? p->L : -1; }
+
+int dwf_lpc_set_weights(dwf_lpc p, const double* W) {
+  CTX();
+  if (!p || !W) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  const int Ls = p->op->Ls;
+  for (int k = 0; k <= p->L; ++k)
+    for (int c = 0; c < 2; ++c)
+      for (int i = 0; i < Ls * Ls; ++i)
+        p->W[k].m[c][i] = W[((size_t)k * 2 + c) * Ls * Ls + i];
+  return lpc_upload(p);
+}
+
+int dwf_lpc_get_weights(dwf_lpc p, double* W) {
+  if (!p || !W) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  const int Ls = p->op->Ls;
+  for (int k = 0; k <= p->L; ++k)
+    for (int c = 0; c < 2; ++c)
+      for (int i = 0; i < Ls * Ls; ++i)
+        W[((size_t)k * 2 + c) * Ls * Ls + i] = (double)p->W[k].m[c][i];
+  return DWF_OK;
+}
+
+int dwf_lpc_apply(dwf_lpc p, const char* which, void* v, const void* w) {
+  CTX();
+  if (!p || !which || !v || !w) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (v == w) return fail(DWF_ERR_INVALID_ARG, "the preconditioner does not work in place");
+  if (!p->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  const std::string m(which);
+  if (m == "K") return lpc_apply_K(p, v, w);
+  if (m == "Kdag") return lpc_apply_Kdag(p, v, w);
+  if (m == "M") return lpc_apply_M(p, v, w);
+  return fail(DWF_ERR_UNKNOWN_MODE, "unknown preconditioner part: " + m);
+}
+
+int dwf_lpc_train(dwf_lpc p, dwf_fopr opt, int nvec, unsigned long long seed, double ridge,
+                  double* loss) {
+  CTX();
+  if (!p || !opt) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (nvec < 1 || !(ridge >= 0)) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  dwf_fopr op = p->op;
+  for (int mu = 0; mu < 4; ++mu)
+    if (opt->N[mu] != op->N[mu]) return fail(DWF_ERR_INVALID_ARG, "lattice sizes differ");
+  if (opt->Ls != op->Ls) return fail(DWF_ERR_INVALID_ARG, "Ls differs");
+  if (!opt->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  const int L = p->L, Ls = op->Ls, F = L + 2;
+  const int NP = F * (F + 1) / 2;
+  const size_t per = 2 * (size_t)Ls * Ls;
+  hipStream_t st = g_ctx.stream;
+  if (!p->gpart) HIPCHK(hipMalloc(&p->gpart, gram_partial_doubles() * sizeof(double)));
+  if (!p->gout) HIPCHK(hipMalloc(&p->gout, 21 * per * sizeof(double)));
+  std::vector<void*> fl(F, nullptr);
+  auto release = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (auto& f : fl)
+      if (f) (void)hipFree(f);
+  };
+  for (int i = 0; i < F; ++i)
+    if (hipMalloc(&fl[i], opt->nelem * prec_bytes(opt->prec)) != hipSuccess) {
+      release();
+      return fail(DWF_ERR_HIP, "device allocation failed");
+    }
+  std::vector<long double> acc(NP * per, 0.0L);
+  std::vector<double> h(NP * per);
+  int rc = DWF_OK;
+  for (int j = 0; j < nvec && rc == DWF_OK; ++j) {
+    // features z_k = H^k D v, target v
+    if (launch_fill_uniform(opt->prec, fl[0], opt->nelem, seed + (unsigned long long)j *
+                                0xD1B54A32D192ED03ull, st) != hipSuccess) {
+      rc = fail(DWF_ERR_HIP, "fill kernel failed");
+      break;
+    }
+    rc = mult_impl(opt, fl[1], fl[0], "D", false);
+    for (int k = 1; k <= L && rc == DWF_OK; ++k) rc = op_Hschur(opt, fl[1 + k], fl[k]);
+    if (rc != DWF_OK) break;
+    if (launch_chiral_gram(opt->prec, fl.data(), F, opt->Vh, Ls, p->gpart, p->gout, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(h.data(), p->gout, NP * per * sizeof(double), hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = fail(DWF_ERR_HIP, "Gram kernel failed");
+      break;
+    }
+    for (size_t i = 0; i < h.size(); ++i) acc[i] += h[i];
+  }
+  release();
+  if (rc != DWF_OK) return rc;
+  // pair index of fields (i <= j)
+  auto pid = [&](int i, int j) { return i * F - i * (i - 1) / 2 + (j - i); };
+  auto G = [&](int i, int j, int c, int a, int b) -> long double {  // sum p(f_i)(a) p(f_j)(b)
+    if (i <= j) return acc[pid(i, j) * per + (size_t)c * Ls * Ls + (size_t)a * Ls + b];
+    return acc[pid(j, i) * per + (size_t)c * Ls * Ls + (size_t)b * Ls + a];
+  };
+  const int d = (L + 1) * Ls;
+  long double lsum = 0, vsum = 0;
+  std::vector<ChiMat> Wn(L + 1, ChiMat(Ls));
+  for (int c = 0; c < 2; ++c) {
+    std::vector<long double> A((size_t)d * d), B((size_t)d * Ls);
+    long double tr = 0;
+    for (int ka = 0; ka <= L; ++ka)
+      for (int a = 0; a < Ls; ++a) {
+        for (int kb = 0; kb <= L; ++kb)
+          for (int b = 0; b < Ls; ++b)
+            A[(size_t)(ka * Ls + a) * d + kb * Ls + b] = G(1 + ka, 1 + kb, c, a, b);
+        tr += A[(size_t)(ka * Ls + a) * d + ka * Ls + a];
+        for (int sidx = 0; sidx < Ls; ++sidx)   // B = Bbig^T: [(k, t)][s] = sum v(s) z_k(t)
+          B[(size_t)(ka * Ls + a) * Ls + sidx] = G(0, 1 + ka, c, sidx, a);
+      }
+    std::vector<long double> A0(A), B0(B);
+    const long double reg = ridge * tr / d;
+    for (int i = 0; i < d; ++i) A[(size_t)i * d + i] += reg;
+    if (!chol_solve(A, d, B, Ls)) return fail(DWF_ERR_NOT_CONVERGED, "training system is singular");
+    // B now holds W^T: [(k, t)][s] = W_k[s][t]
+    long double vv = 0;
+    for (int sidx = 0; sidx < Ls; ++sidx) vv += G(0, 0, c, sidx, sidx);
+    // |W Z - V|^2 = vv - 2 tr(W Bbig^T) + tr(W G W^T)
+    long double wb = 0, wgw = 0;
+    for (int i = 0; i < d; ++i)
+      for (int sidx = 0; sidx < Ls; ++sidx) {
+        wb += B[(size_t)i * Ls + sidx] * B0[(size_t)i * Ls + sidx];
+        long double gw = 0;
+        for (int j2 = 0; j2 < d; ++j2) gw += A0[(size_t)i * d + j2] * B[(size_t)j2 * Ls + sidx];
+        wgw += B[(size_t)i * Ls + sidx] * gw;
+      }
+    lsum += 0.5L * (vv - 2 * wb + wgw);
+    vsum += 0.5L * vv;
+    for (int k = 0; k <= L; ++k)
+      for (int sidx = 0; sidx < Ls; ++sidx)
+        for (int t = 0; t < Ls; ++t) Wn[k].at(c, sidx, t) = B[(size_t)(k * Ls + t) * Ls + sidx];
+  }
+  p->W = Wn;
+  p->loss = (double)(lsum / vsum);
+  if (loss) *loss = p->loss;
+  return lpc_upload(p);
+}
+
+int dwf_lpc_chiral_gram(dwf_fopr op, int nfields, const void* const* fields, double* out_host) {
+  CTX();
+  if (!op || !fields || !out_host || nfields < 1 || nfields > 6)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (op->Ls > 16) return fail(DWF_ERR_UNSUPPORTED, "Ls > 16");
+  const size_t per = 2 * (size_t)op->Ls * op->Ls;
+  const int NP = nfields * (nfields + 1) / 2;
+  double *part = nullptr, *o = nullptr;
+  HIPCHK(hipMalloc(&part, gram_partial_doubles() * sizeof(double)));
+  int rc = DWF_OK;
+  if (hipMalloc(&o, NP * per * sizeof(double)) != hipSuccess)
+    rc = fail(DWF_ERR_HIP, "device allocation failed");
+  if (rc == DWF_OK &&
+      (launch_chiral_gram(op->prec, fields, nfields, op->Vh, op->Ls, part, o, g_ctx.stream) !=
+           hipSuccess ||
+       hipMemcpyAsync(out_host, o, NP * per * sizeof(double), hipMemcpyDeviceToHost,
+                      g_ctx.stream) != hipSuccess ||
+       hipStreamSynchronize(g_ctx.stream) != hipSuccess))
+    rc = fail(DWF_ERR_HIP, "Gram kernel failed");
+  (void)hipStreamSynchronize(g_ctx.stream);
+  (void)hipFree(part);
+  if (o) (void)hipFree(o);
+  return rc;
+}
+
+int dwf_field_fill_uniform(dwf_fopr op, void* x, unsigned long long seed) {
+  CTX();
+  if (!op || !x) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_fill_uniform(op->prec, x, op->nelem, seed, g_ctx.stream));
   return DWF_OK;
 }
 

@@ -34,6 +34,8 @@ struct Epi {
   double xcoef = 0.0;
   void* out2 = nullptr;
   const void* F = nullptr;
+  const void* G = nullptr;    // with X: out = E core + G X (chiral matrix) instead of + xcoef X
+  const void* X2 = nullptr;   // with out2: out2 = F out + X2 (X2 may alias out2)
 };
 
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
@@ -102,6 +104,16 @@ hipError_t launch_force_tensor(double* F, const double* Z, const double* Vn, con
 // F <- -2 at(U F) with U from the packed forward links
 hipError_t launch_mult_generator(double* F, const double* links, int Nx, int Ny, int Nz, int Nt,
                                  const int* bc, hipStream_t st);
+
+// Learned preconditioner: x_i = uniform [-1, 1) from a counter hash of (seed, i)
+hipError_t launch_fill_uniform(int prec, void* x, size_t n, unsigned long long seed,
+                               hipStream_t st);
+// Chiral Gram matrices of F <= 6 fields (Ls <= 16), all pairs i <= j (row-major
+// over i): out[p][chi][s][t] = sum_{site, k < 12} p_chi(f_i)(s) p_chi(f_j)(t),
+// p_+ = f_k + f_{k+12}, p_- = f_k - f_{k+12}.  partial: gram_partial_doubles().
+hipError_t launch_chiral_gram(int prec, const void* const* fields, int F, int Vh, int Ls,
+                              double* partial, double* out, hipStream_t st);
+size_t gram_partial_doubles();
 
 // Gauge: one stout smearing level, lexical Field_G (4, V, 18) double device.
 hipError_t launch_stout(double* Uout, const double* Uin, int Nx, int Ny, int Nz, int Nt,

@@ -181,7 +181,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
       const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
       const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
-      const typename St<T>::C* __restrict__ F) {
+      const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
+      const T* X2) {
   using S = St<T>;
   using C = typename S::C;
   const int Ls = LSC > 0 ? LSC : Ls_rt;
@@ -198,6 +199,9 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   C* sh_l = shA + spb * 12 * Ls;
   C* sh_E = shA + regionA;
   C* sh_F = sh_E + 2 * Ls * Ls;
+  C* sh_G = sh_F + (OUT2 ? 2 * Ls * Ls : 0);
+  // G != 0: the added field X enters through its own chiral matrix (uniform)
+  const bool xmat = ADDX && MAT && G != nullptr;
 
   const int tid = threadIdx.x;
   const int lsite = tid / Ls;
@@ -215,6 +219,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const int so = r / Ls, si = r - so * Ls;
       sh_E[chi * LL + si * Ls + so] = E[i];
       if (OUT2) sh_F[chi * LL + si * Ls + so] = F[i];
+      if (xmat) sh_G[chi * LL + si * Ls + so] = G[i];
     }
   }
   if (OP != K_SITE) {
@@ -292,14 +297,13 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     }
   }
   __syncthreads();
-  C o[24];
-  if (valid) {
-    C ou[12], ol[12];
+  const int LL = Ls * Ls;
+  C ou[12], ol[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) { ou[k] = 0; ol[k] = 0; }
+  for (int k = 0; k < 12; ++k) { ou[k] = 0; ol[k] = 0; }
+  if (valid) {
     const C* u = sh_u + lsite * 12 * Ls;
     const C* l = sh_l + lsite * 12 * Ls;
-    const int LL = Ls * Ls;
     for (int s = 0; s < Ls; ++s) {
       const C ep = sh_E[s * Ls + ls];
       const C em = sh_E[LL + s * Ls + ls];
@@ -309,12 +313,43 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         ol[k] += em * l[k * Ls + s];
       }
     }
+  }
+  if (xmat) {
+    // + G X: stage the chiral components of X in the same LDS rows
+    __syncthreads();
+    if (valid) {
+      C* u = sh_u + lsite * 12 * Ls + ls;
+      C* l = sh_l + lsite * 12 * Ls + ls;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        const C a = S::ld(X + base + k * Ls), b = S::ld(X + base + (k + 12) * Ls);
+        u[k * Ls] = a + b;
+        l[k * Ls] = a - b;
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      const C* u = sh_u + lsite * 12 * Ls;
+      const C* l = sh_l + lsite * 12 * Ls;
+      for (int s = 0; s < Ls; ++s) {
+        const C gp = sh_G[s * Ls + ls];
+        const C gm = sh_G[LL + s * Ls + ls];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          ou[k] += gp * u[k * Ls + s];
+          ol[k] += gm * l[k * Ls + s];
+        }
+      }
+    }
+  }
+  C o[24];
+  if (valid) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       o[k] = ou[k] + ol[k];
       o[k + 12] = ou[k] - ol[k];
     }
-    if (ADDX) {
+    if (ADDX && !xmat) {
 #pragma unroll
       for (int k = 0; k < 24; ++k) o[k] += xcoef * S::ld(X + base + k * Ls);
     }
@@ -334,12 +369,10 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     }
     __syncthreads();
     if (valid) {
-      C ou[12], ol[12];
 #pragma unroll
       for (int k = 0; k < 12; ++k) { ou[k] = 0; ol[k] = 0; }
       const C* u = sh_u + lsite * 12 * Ls;
       const C* l = sh_l + lsite * 12 * Ls;
-      const int LL = Ls * Ls;
       for (int s = 0; s < Ls; ++s) {
         const C fp = sh_F[s * Ls + ls];
         const C fm = sh_F[LL + s * Ls + ls];
@@ -347,6 +380,15 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         for (int k = 0; k < 12; ++k) {
           ou[k] += fp * u[k * Ls + s];
           ol[k] += fm * l[k * Ls + s];
+        }
+      }
+      // X2 != 0: out2 = F o + X2 (X2 may be out2 itself: each element is read
+      // and written by this thread only)
+      if (X2) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          ou[k] += (C)0.5 * (S::ld(X2 + base + k * Ls) + S::ld(X2 + base + (k + 12) * Ls));
+          ol[k] += (C)0.5 * (S::ld(X2 + base + k * Ls) - S::ld(X2 + base + (k + 12) * Ls));
         }
       }
 #pragma unroll
@@ -365,7 +407,8 @@ static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* ou
   using C = typename St<T>::C;
   hipLaunchKernelGGL((k_dwf<T, OP, MAT, ADDX, OUT2, LSC>), grid, block, lds, st, (T*)out,
                      (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
-                     (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F);
+                     (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F,
+                     (const C*)e.G, (const T*)e.X2);
 }
 
 // the (op, epilogue) combinations the operator chains use
@@ -385,16 +428,18 @@ static hipError_t launch_combo(int op, bool mat, bool addx, bool out2, dim3 g, d
       if (addx || out2) return hipErrorInvalidValue;
       LK(K_HOP, false, false, false);
     } else if (!addx) {
-      if (out2) return hipErrorInvalidValue;
-      LK(K_HOP, true, false, false);
+      if (out2) LK(K_HOP, true, false, true);
+      else LK(K_HOP, true, false, false);
     } else if (out2) {
       LK(K_HOP, true, true, true);
     } else {
       LK(K_HOP, true, true, false);
     }
   } else {
-    if (!mat || out2) return hipErrorInvalidValue;
-    if (addx) LK(K_HOPDAG, true, true, false);
+    if (!mat) return hipErrorInvalidValue;
+    if (addx && out2) LK(K_HOPDAG, true, true, true);
+    else if (addx) LK(K_HOPDAG, true, true, false);
+    else if (out2) return hipErrorInvalidValue;
     else LK(K_HOPDAG, true, false, false);
   }
 #undef LK
@@ -413,7 +458,9 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   int spb = DWF_HOP_THREADS / Ls;
   if (spb < 1) return hipErrorInvalidValue;
   const size_t cs = sizeof(C);
-  const size_t matb = mat ? 2 * (size_t)Ls * Ls * cs * (out2 ? 2 : 1) : 0;
+  if ((e.G && (!mat || !addx)) || (e.X2 && !out2)) return hipErrorInvalidValue;
+  const size_t matb =
+      mat ? 2 * (size_t)Ls * Ls * cs * (1 + (out2 ? 1 : 0) + (e.G ? 1 : 0)) : 0;
   const size_t lds_cap = 64 * 1024;
   auto lds_for = [&](int sp) {
     const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * sp * cs;

@@ -288,6 +288,99 @@ class ASolver_Richardson(ASolver_CG):
         self.fopr_f = fopr_f
 
 
+class APrecond_LearnedPolynomial:
+    """Learned polynomial preconditioner K = sum_k W_k H^k, H = 1 - D (no
+    counterpart in the reference).  Applied in fopr's precision; trained by
+    least squares with any operator of the same lattice and parameters."""
+
+    def __init__(self, fopr, degree):
+        h = C.c_void_p()
+        _lib.check("dwf_lpc_create", _lib.load().dwf_lpc_create(fopr._h, int(degree), C.byref(h)))
+        self._h = h.value
+        self.fopr = fopr
+        self.degree = int(degree)
+        self.loss = None
+
+    @property
+    def weights_shape(self):
+        return (self.degree + 1, 2, self.fopr.Ls, self.fopr.Ls)
+
+    def get_weights(self):
+        W = np.zeros(self.weights_shape)
+        _lib.check("dwf_lpc_get_weights", _lib.load().dwf_lpc_get_weights(self._h, _ptr(W)))
+        return W
+
+    def set_weights(self, W):
+        W = np.ascontiguousarray(W, np.float64)
+        assert W.shape == self.weights_shape
+        _lib.check("dwf_lpc_set_weights", _lib.load().dwf_lpc_set_weights(self._h, _ptr(W)))
+
+    def train(self, fopr_train, nvec=2, seed=12345, ridge=1e-10):
+        loss = np.zeros(1)
+        _lib.check("dwf_lpc_train", _lib.load().dwf_lpc_train(
+            self._h, fopr_train._h, int(nvec), int(seed), float(ridge), _ptr(loss)))
+        self.loss = float(loss[0])
+        return self.loss
+
+    def apply(self, which, v, w):
+        _lib.check("dwf_lpc_apply", _lib.load().dwf_lpc_apply(self._h, which.encode(), v.ptr, w.ptr))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.load().dwf_lpc_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def chiral_gram(fopr, fields):
+    """Chiral Gram matrices of up to 6 fields (all pairs i <= j)."""
+    F = len(fields)
+    NP = F * (F + 1) // 2
+    out = np.zeros((NP, 2, fopr.Ls, fopr.Ls))
+    arr = (C.c_void_p * F)(*[f.ptr for f in fields])
+    _lib.check("dwf_lpc_chiral_gram", _lib.load().dwf_lpc_chiral_gram(fopr._h, F, arr, _ptr(out)))
+    return out
+
+
+def fill_uniform(field, seed):
+    _lib.check("dwf_field_fill_uniform",
+               _lib.load().dwf_field_fill_uniform(field.fopr._h, field.ptr, int(seed)))
+
+
+class ASolver_FCG(ASolver_CG):
+    """Flexible CG on fopr (mode DdagD) preconditioned by M = K K^+."""
+
+    def __init__(self, fopr, prec, maximum_number_of_iteration, convergence_criterion_squared):
+        h = C.c_void_p()
+        _lib.check("dwf_solver_create_fcg", _lib.load().dwf_solver_create_fcg(
+            fopr._h, prec._h, int(maximum_number_of_iteration),
+            float(convergence_criterion_squared), C.byref(h)))
+        self._h = h.value
+        self.fopr = fopr
+        self.prec = prec
+
+
+class ASolver_Richardson_LPC(ASolver_CG):
+    """Richardson on fopr_d with an inner flexible CG on fopr_f preconditioned
+    by the learned preconditioner prec."""
+
+    def __init__(self, fopr_d, fopr_f, prec, maximum_number_of_iteration,
+                 convergence_criterion_squared, inner_iteration=100000,
+                 inner_convergence_criterion_squared=1.0e-8):
+        h = C.c_void_p()
+        _lib.check("dwf_solver_create_richardson_lpc",
+                   _lib.load().dwf_solver_create_richardson_lpc(
+                       fopr_d._h, fopr_f._h, prec._h, int(maximum_number_of_iteration),
+                       float(convergence_criterion_squared), int(inner_iteration),
+                       float(inner_convergence_criterion_squared), C.byref(h)))
+        self._h = h.value
+        self.fopr = fopr_d
+        self.fopr_f = fopr_f
+        self.prec = prec
+
+
 class AShiftsolver_CG:
     """Multishift CG (ashiftsolver_CG-tmpl.h) on fopr.mult + sigma_i."""
 

@@ -100,6 +100,18 @@ _SIGS = {
     "dwf_solver_solve": (C.c_int, [_vp, _vp, _vp, _ip, _dp]),
     "dwf_solver_inner_iterations": (C.c_int, [_vp]),
     "dwf_solver_destroy": (C.c_int, [_vp]),
+    "dwf_lpc_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "dwf_lpc_destroy": (C.c_int, [_vp]),
+    "dwf_lpc_degree": (C.c_int, [_vp]),
+    "dwf_lpc_set_weights": (C.c_int, [_vp, _dp]),
+    "dwf_lpc_get_weights": (C.c_int, [_vp, _dp]),
+    "dwf_lpc_train": (C.c_int, [_vp, _vp, C.c_int, C.c_ulonglong, C.c_double, _dp]),
+    "dwf_lpc_apply": (C.c_int, [_vp, C.c_char_p, _vp, _vp]),
+    "dwf_lpc_chiral_gram": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), _dp]),
+    "dwf_field_fill_uniform": (C.c_int, [_vp, _vp, C.c_ulonglong]),
+    "dwf_solver_create_fcg": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.POINTER(_vp)]),
+    "dwf_solver_create_richardson_lpc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
+                                                   C.c_double, C.POINTER(_vp)]),
     "dwf_shiftsolver_cg_solve": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _dp,
                                            C.POINTER(_vp), _vp, _ip, _dp]),
     "dwf_force_create": (C.c_int, [_vp, C.POINTER(_vp)]),

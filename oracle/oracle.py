@@ -213,6 +213,107 @@ def rational_ratio_force_udiv(orc1, orc2, rat1, rat2, eta, niter=10000, stop_con
     return F
 
 
+# ---- learned polynomial preconditioner (no reference counterpart) ----------
+# K = sum_k W_k H^k, H = 1 - D; W_k = (W+, W-) per-chirality Ls x Ls matrices
+# acting on P+ x and P- x (Dirac basis: P+- x = (x_k +- x_{k+12}) / 2 on both halves).
+
+def chiral_apply(W, x):
+    """(W x) = W+ P+ x + W- P- x for a corelib field x (Ls, Vh, 24); W (2, Ls, Ls)."""
+    u = x[..., :12] + x[..., 12:]
+    l = x[..., :12] - x[..., 12:]
+    ou = np.einsum("st,tvk->svk", W[0], u)
+    ol = np.einsum("st,tvk->svk", W[1], l)
+    out = np.empty_like(x)
+    out[..., :12] = 0.5 * (ou + ol)
+    out[..., 12:] = 0.5 * (ou - ol)
+    return out
+
+
+def lpc_K(orc, W, x):
+    acc = chiral_apply(W[0], x)
+    z = x
+    for k in range(1, len(W)):
+        z = z - orc.mult("D", z)
+        acc = acc + chiral_apply(W[k], z)
+    return acc
+
+
+def lpc_Kdag(orc, W, x):
+    L = len(W) - 1
+    Wt = [np.transpose(w, (0, 2, 1)) for w in W]
+    t = chiral_apply(Wt[L], x)
+    for k in range(L - 1, -1, -1):
+        t = chiral_apply(Wt[k], x) + (t - orc.mult_dag("D", t))
+    return t
+
+
+def chiral_gram(fields):
+    """All pairs i <= j: G[p][chi][s][t] = sum p_chi(f_i)(s) p_chi(f_j)(t)."""
+    P = [(f[..., :12] + f[..., 12:], f[..., :12] - f[..., 12:]) for f in fields]
+    out = []
+    for i in range(len(fields)):
+        for j in range(i, len(fields)):
+            out.append([np.einsum("svk,tvk->st", P[i][c], P[j][c]) for c in (0, 1)])
+    return np.array(out)
+
+
+_M64 = (1 << 64) - 1
+
+
+def hash_uniform(seed, idx):
+    """x = uniform [-1, 1) of the counter hash (splitmix64) of (seed, idx)."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & _M64) + (idx + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -52 - 1.0
+
+
+def uniform_field(seed, Vh, Ls):
+    """The device fill of (seed) in the corelib layout (Ls, Vh, 24): device
+    index ((site * 24) + k) * Ls + s."""
+    s = np.arange(Ls, dtype=np.uint64)[:, None, None]
+    site = np.arange(Vh, dtype=np.uint64)[None, :, None]
+    k = np.arange(24, dtype=np.uint64)[None, None, :]
+    return hash_uniform(seed, (site * np.uint64(24) + k) * np.uint64(Ls) + s)
+
+
+def lpc_train(orc, L, vs, ridge=0.0):
+    """Least-squares weights of K (degree L) with K D v ~ v over the fields vs;
+    returns (W, relative loss)."""
+    Ls = orc.Ls
+    d = (L + 1) * Ls
+    A = np.zeros((2, d, d))
+    B = np.zeros((2, d, Ls))
+    vv = np.zeros(2)
+    for v in vs:
+        z = [orc.mult("D", v)]
+        for _ in range(L):
+            z.append(z[-1] - orc.mult("D", z[-1]))
+        G = chiral_gram([v] + z)
+        F = L + 2
+        pid = lambda i, j: i * F - i * (i - 1) // 2 + (j - i)
+        for c in (0, 1):
+            for a in range(L + 1):
+                for b in range(L + 1):
+                    i, j = 1 + a, 1 + b
+                    blk = G[pid(i, j)][c] if i <= j else G[pid(j, i)][c].T
+                    A[c, a * Ls:(a + 1) * Ls, b * Ls:(b + 1) * Ls] += blk
+                B[c, a * Ls:(a + 1) * Ls, :] += G[pid(0, 1 + a)][c].T
+            vv[c] += np.trace(G[pid(0, 0)][c])
+    W = np.zeros((L + 1, 2, Ls, Ls))
+    lsum = 0.0
+    for c in (0, 1):
+        Ar = A[c] + ridge * np.trace(A[c]) / d * np.eye(d)
+        X = np.linalg.solve(Ar, B[c])            # X[(k, t)][s] = W_k[s][t]
+        lsum += 0.5 * (vv[c] - 2 * np.sum(X * B[c]) + np.sum(X * (A[c] @ X)))
+        for k in range(L + 1):
+            W[k, c] = X[k * Ls:(k + 1) * Ls, :].T
+    return W, lsum / (0.5 * vv.sum())
+
+
 def richardson(op64, op32, b, mode="DdagD", niter_outer=100, stop_outer2=1e-28,
                niter_inner=10000, stop_inner2=1e-8):
     b = np.ascontiguousarray(b, np.float64)
