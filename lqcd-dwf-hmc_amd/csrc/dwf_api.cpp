@@ -853,7 +853,7 @@ struct dwf_lpc_s {
   double loss = -1.0;
 };
 
-// device matrix slots: 0 Q = P(-E2); 1 W_0; 1+k A_k = W_k P^-1 (k = 1..L);
+// device matrix slots: 0 Q = P(-E2); 1+k A_k = W_k P^-1 (k = 0..L);
 // L+2 Di^T W_L^T; L+3+k W_k^T (k = 0..L-1)
 static const void* lpc_mat(const dwf_lpc_s* p, int slot) {
   return (const char*)p->mats + (size_t)slot * 2 * p->op->Ls * p->op->Ls * cbytes(p->op->prec);
@@ -864,7 +864,7 @@ static int lpc_upload(dwf_lpc_s* p) {
   const int L = p->L;
   std::vector<ChiMat> ms;
   ms.push_back(cm_mul(p->P, op->hmats[M_HE2]));
-  ms.push_back(p->W[0]);
+  ms.push_back(cm_mul(p->W[0], p->Pinv));   // applied to Mp w (the second output of step 0)
   for (int k = 1; k <= L; ++k) ms.push_back(cm_mul(p->W[k], p->Pinv));
   ms.push_back(cm_mul(op->hmats[M_DIT], cm_transpose(p->W[L])));
   for (int k = 0; k < L; ++k) ms.push_back(cm_transpose(p->W[k]));
@@ -892,7 +892,6 @@ static int site_m(dwf_fopr op, void* out, const void* in, const void* E, void* o
 static int lpc_apply_K(dwf_lpc_s* p, void* v, const void* w) {
   dwf_fopr op = p->op;
   const int L = p->L;
-  if (L == 0) return site_m(op, v, w, lpc_mat(p, 1));
   void *zh, *t1, *tmp;
   RC(lpc_field(p, 0, &zh));
   RC(lpc_field(p, 1, &t1));
@@ -902,7 +901,7 @@ static int lpc_apply_K(dwf_lpc_s* p, void* v, const void* w) {
     RC(site_m(op, zh, w, mat(op, M_PRE), v, lpc_mat(p, 1)));   // zh = Mp w, v = W_0 w
     zin = zh;
   } else {
-    RC(site_m(op, v, w, lpc_mat(p, 1)));
+    RC(site_m(op, v, w, lpc_mat(p, 1)));   // P = 1: A_0 = W_0
   }
   for (int k = 1; k <= L; ++k) {
     RC(run_hop(op, false, 1, t1, zin, M_E1));
@@ -922,7 +921,6 @@ static int lpc_apply_K(dwf_lpc_s* p, void* v, const void* w) {
 static int lpc_apply_Kdag(dwf_lpc_s* p, void* v, const void* w) {
   dwf_fopr op = p->op;
   const int L = p->L;
-  if (L == 0) return site_m(op, v, w, lpc_mat(p, L + 3));
   void *th, *t1, *tmp;
   RC(lpc_field(p, 0, &th));
   RC(lpc_field(p, 1, &t1));

@@ -118,36 +118,43 @@ __device__ __forceinline__ void su3_mul(const C* u, const C* h, C* a) {
   }
 }
 
-// One hop direction for thread (site, ls).  The block's links (already in
-// the compute type) and neighbour indices sit in LDS: shL[(dir*18 + e)*spb + lsite],
-// shN[dir*spb + lsite].
-template <int DIR, bool DAG, typename T, typename C>
-__device__ __forceinline__ void hop_dir(C* acc, const T* __restrict__ in, const C* shL,
-                                        const int* shN, int lsite, int ls, int spb, int Ls) {
-  using S = St<T>;
+// One hop direction for thread (site, s0 .. s0+VEC-1).  The block's links
+// (already in the compute type) and neighbour indices sit in LDS:
+// shL[(dir*18 + e)*spb + lsite], shN[dir*spb + lsite].  The VEC s-values of a
+// spinor component are one vector load; the link is read once for all VEC.
+template <int DIR, bool DAG, typename T, int VEC, typename C>
+__device__ __forceinline__ void hop_dir(C (*acc)[24], const T* __restrict__ in, const C* shL,
+                                        const int* shN, int lsite, int s0, int spb, int Ls) {
+  using VL = Vx<T, VEC>;
   const int nb = shN[DIR * spb + lsite];
-  const T* wp = in + (size_t)nb * 24 * Ls + ls;
-  C w[24];
+  const T* wp = in + (size_t)nb * 24 * Ls + s0;
+  typename VL::R raw[24];
   if (!DAG) {
 #pragma unroll
-    for (int k = 0; k < 24; ++k) w[k] = S::ld(wp + k * Ls);
+    for (int k = 0; k < 24; ++k) raw[k] = VL::ld(wp + k * Ls);
   } else {  // gamma_5 (Dirac basis) exchanges the upper and lower spin pairs
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      w[k] = S::ld(wp + (k + 12) * Ls);
-      w[k + 12] = S::ld(wp + k * Ls);
+      raw[k] = VL::ld(wp + (k + 12) * Ls);
+      raw[k + 12] = VL::ld(wp + k * Ls);
     }
   }
-  C h1[6], h2[6];
-  project<DIR>(w, h1, h2);
   const C* up = shL + DIR * 18 * spb + lsite;
   C u[18];
 #pragma unroll
   for (int e = 0; e < 18; ++e) u[e] = up[e * spb];
-  C a1[6], a2[6];
-  su3_mul(u, h1, a1);
-  su3_mul(u, h2, a2);
-  reconstruct<DIR>(acc, a1, a2);
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    C w[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) w[k] = VL::get(raw[k], v);
+    C h1[6], h2[6];
+    project<DIR>(w, h1, h2);
+    C a1[6], a2[6];
+    su3_mul(u, h1, a1);
+    su3_mul(u, h2, a2);
+    reconstruct<DIR>(acc[v], a1, a2);
+  }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -167,16 +174,41 @@ __device__ __forceinline__ int xcd_block(int w, int G) {
 #ifndef DWF_WAVES_F32
 #define DWF_WAVES_F32 4
 #endif
+#ifndef DWF_WAVES_F16
+#define DWF_WAVES_F16 3
+#endif
 #ifndef DWF_HOP_THREADS
 #define DWF_HOP_THREADS 256
 #endif
-template <typename T> struct HopWaves { static constexpr int value = DWF_WAVES_F32; };
-template <> struct HopWaves<double> { static constexpr int value = DWF_WAVES_F64; };
+#ifndef DWF_VEC_F32
+#define DWF_VEC_F32 1
+#endif
+#ifndef DWF_VEC_F16
+#define DWF_VEC_F16 2
+#endif
+// Per storage type: s-values per thread (VEC, one vector access per spinor
+// component), threads per block and the occupancy target.
+template <typename T> struct KCfg {   // 16-bit storage
+  static constexpr int VEC = DWF_VEC_F16;
+  static constexpr int THREADS = 128;
+  static constexpr int WAVES = DWF_WAVES_F16;
+};
+template <> struct KCfg<float> {
+  static constexpr int VEC = DWF_VEC_F32;
+  static constexpr int THREADS = DWF_HOP_THREADS;
+  static constexpr int WAVES = DWF_WAVES_F32;
+};
+template <> struct KCfg<double> {
+  static constexpr int VEC = 1;
+  static constexpr int THREADS = DWF_HOP_THREADS;
+  static constexpr int WAVES = DWF_WAVES_F64;
+};
 
-// LSC > 0: Ls and the block shape (256/LSC sites) are compile-time constants,
-// so every load and LDS access uses an immediate offset; LSC = 0: runtime Ls.
-template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC>
-__global__ void __launch_bounds__(DWF_HOP_THREADS, HopWaves<T>::value)
+// LSC > 0: Ls and the block shape (THREADS / (LSC / VEC) sites) are compile-time
+// constants, so every load and LDS access uses an immediate offset; LSC = 0:
+// runtime Ls.  Thread (lsite, q) owns s = q*VEC .. q*VEC + VEC - 1 of its site.
+template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC, int VEC>
+__global__ void __launch_bounds__(KCfg<T>::THREADS, KCfg<T>::WAVES)
 k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links,
       const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
       const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
@@ -185,18 +217,23 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const T* X2) {
   using S = St<T>;
   using C = typename S::C;
+  using VL = Vx<T, VEC>;
   const int Ls = LSC > 0 ? LSC : Ls_rt;
-  const int spb = LSC > 0 ? DWF_HOP_THREADS / LSC : spb_rt;
+  const int Q = Ls / VEC;   // threads per site
+  const int spb = LSC > 0 ? KCfg<T>::THREADS / (LSC / VEC) : spb_rt;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // region A: hop staging (links, neighbours), later the chiral epilogue buffers
+  // chiral rows of one site are 12*Ls + 1 apart, so the sites of a wave read
+  // their rows from different LDS banks
+  const int SU = 12 * Ls + 1;
   const int stageC = (OP == K_SITE) ? 0 : (144 * spb + 8 * spb);
-  const int epiC = MAT ? 24 * spb * Ls : 0;
+  const int epiC = MAT ? 2 * spb * SU : 0;
   const int regionA = stageC > epiC ? stageC : epiC;
   C* shA = reinterpret_cast<C*>(smem_raw);
   C* shL = shA;
   int* shN = reinterpret_cast<int*>(shA + 144 * spb);
   C* sh_u = shA;
-  C* sh_l = shA + spb * 12 * Ls;
+  C* sh_l = shA + spb * SU;
   C* sh_E = shA + regionA;
   C* sh_F = sh_E + 2 * Ls * Ls;
   C* sh_G = sh_F + (OUT2 ? 2 * Ls * Ls : 0);
@@ -204,8 +241,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   const bool xmat = ADDX && MAT && G != nullptr;
 
   const int tid = threadIdx.x;
-  const int lsite = tid / Ls;
-  const int ls = tid - lsite * Ls;
+  const int lsite = tid / Q;
+  const int s0 = (tid - lsite * Q) * VEC;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int site0 = blk * spb;
   const int site = site0 + lsite;
@@ -237,49 +274,62 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     __syncthreads();
   }
 
-  C acc[24];
+  C acc[VEC][24];
 #pragma unroll
-  for (int k = 0; k < 24; ++k) acc[k] = 0;
+  for (int v = 0; v < VEC; ++v)
+#pragma unroll
+    for (int k = 0; k < 24; ++k) acc[v][k] = 0;
+  const size_t base = (size_t)site * 24 * Ls + s0;
   if (valid) {
     if (OP == K_SITE) {
-      const T* wp = in + (size_t)site * 24 * Ls + ls;
 #pragma unroll
-      for (int k = 0; k < 24; ++k) acc[k] = S::ld(wp + k * Ls);
-    } else if (OP == K_HOP) {
-      hop_dir<0, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<1, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<2, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<3, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<4, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<5, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<6, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<7, false, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-    } else {
-      hop_dir<0, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<1, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<2, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<3, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<4, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<5, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<6, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
-      hop_dir<7, true, T>(acc, in, shL, shN, lsite, ls, spb, Ls);
+      for (int k = 0; k < 24; ++k) {
+        const typename VL::R r = VL::ld(in + base + k * Ls);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        C t = acc[k];
-        acc[k] = acc[k + 12];
-        acc[k + 12] = t;
+        for (int v = 0; v < VEC; ++v) acc[v][k] = VL::get(r, v);
       }
+    } else if (OP == K_HOP) {
+      hop_dir<0, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<1, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<2, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<3, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<4, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<5, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<6, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<7, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+    } else {
+      hop_dir<0, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<1, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<2, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<3, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<4, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<5, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<6, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_dir<7, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          C t = acc[v][k];
+          acc[v][k] = acc[v][k + 12];
+          acc[v][k + 12] = t;
+        }
     }
   }
 
-  const size_t base = (size_t)site * 24 * Ls + ls;
   if (!MAT) {
     if (valid) {
 #pragma unroll
       for (int k = 0; k < 24; ++k) {
-        C v = scale * acc[k];
-        if (ADDX) v += xcoef * S::ld(X + base + k * Ls);
-        S::stn(out + base + k * Ls, v);
+        C o[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) o[v] = scale * acc[v][k];
+        if (ADDX) {
+          const typename VL::R r = VL::ld(X + base + k * Ls);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) o[v] += xcoef * VL::get(r, v);
+        }
+        VL::stn(out + base + k * Ls, o);
       }
     }
     return;
@@ -288,29 +338,42 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   if (OP != K_SITE) __syncthreads();  // staging area is reused below
   // chiral components of the core, per (site, k < 12, s): P+ ~ u, P- ~ l
   if (valid) {
-    C* u = sh_u + lsite * 12 * Ls + ls;
-    C* l = sh_l + lsite * 12 * Ls + ls;
+    C* u = sh_u + lsite * SU + s0;
+    C* l = sh_l + lsite * SU + s0;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      u[k * Ls] = acc[k] + acc[k + 12];
-      l[k * Ls] = acc[k] - acc[k + 12];
-    }
+    for (int k = 0; k < 12; ++k)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        u[k * Ls + v] = acc[v][k] + acc[v][k + 12];
+        l[k * Ls + v] = acc[v][k] - acc[v][k + 12];
+      }
   }
   __syncthreads();
   const int LL = Ls * Ls;
-  C ou[12], ol[12];
+  // ou[v][k] = sum_s E+[s0+v][s] u[k][s] (and E- on l); E sits in LDS as [chi][s][s_out]
+  C ou[VEC][12], ol[VEC][12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) { ou[k] = 0; ol[k] = 0; }
+  for (int v = 0; v < VEC; ++v)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) { ou[v][k] = 0; ol[v][k] = 0; }
   if (valid) {
-    const C* u = sh_u + lsite * 12 * Ls;
-    const C* l = sh_l + lsite * 12 * Ls;
+    const C* u = sh_u + lsite * SU;
+    const C* l = sh_l + lsite * SU;
     for (int s = 0; s < Ls; ++s) {
-      const C ep = sh_E[s * Ls + ls];
-      const C em = sh_E[LL + s * Ls + ls];
+      C ep[VEC], em[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        ep[v] = sh_E[s * Ls + s0 + v];
+        em[v] = sh_E[LL + s * Ls + s0 + v];
+      }
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
-        ou[k] += ep * u[k * Ls + s];
-        ol[k] += em * l[k * Ls + s];
+        const C us = u[k * Ls + s], ls_ = l[k * Ls + s];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          ou[v][k] += ep[v] * us;
+          ol[v][k] += em[v] * ls_;
+        }
       }
     }
   }
@@ -318,68 +381,104 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     // + G X: stage the chiral components of X in the same LDS rows
     __syncthreads();
     if (valid) {
-      C* u = sh_u + lsite * 12 * Ls + ls;
-      C* l = sh_l + lsite * 12 * Ls + ls;
+      C* u = sh_u + lsite * SU + s0;
+      C* l = sh_l + lsite * SU + s0;
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
-        const C a = S::ld(X + base + k * Ls), b = S::ld(X + base + (k + 12) * Ls);
-        u[k * Ls] = a + b;
-        l[k * Ls] = a - b;
+        const typename VL::R ra = VL::ld(X + base + k * Ls);
+        const typename VL::R rb = VL::ld(X + base + (k + 12) * Ls);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const C a = VL::get(ra, v), b = VL::get(rb, v);
+          u[k * Ls + v] = a + b;
+          l[k * Ls + v] = a - b;
+        }
       }
     }
     __syncthreads();
     if (valid) {
-      const C* u = sh_u + lsite * 12 * Ls;
-      const C* l = sh_l + lsite * 12 * Ls;
+      const C* u = sh_u + lsite * SU;
+      const C* l = sh_l + lsite * SU;
       for (int s = 0; s < Ls; ++s) {
-        const C gp = sh_G[s * Ls + ls];
-        const C gm = sh_G[LL + s * Ls + ls];
+        C gp[VEC], gm[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          gp[v] = sh_G[s * Ls + s0 + v];
+          gm[v] = sh_G[LL + s * Ls + s0 + v];
+        }
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-          ou[k] += gp * u[k * Ls + s];
-          ol[k] += gm * l[k * Ls + s];
+          const C us = u[k * Ls + s], ls_ = l[k * Ls + s];
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            ou[v][k] += gp[v] * us;
+            ol[v][k] += gm[v] * ls_;
+          }
         }
       }
     }
   }
-  C o[24];
+  C o[VEC][24];
   if (valid) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      o[k] = ou[k] + ol[k];
-      o[k + 12] = ou[k] - ol[k];
-    }
+    for (int v = 0; v < VEC; ++v)
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        o[v][k] = ou[v][k] + ol[v][k];
+        o[v][k + 12] = ou[v][k] - ol[v][k];
+      }
     if (ADDX && !xmat) {
 #pragma unroll
-      for (int k = 0; k < 24; ++k) o[k] += xcoef * S::ld(X + base + k * Ls);
+      for (int k = 0; k < 24; ++k) {
+        const typename VL::R r = VL::ld(X + base + k * Ls);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) o[v][k] += xcoef * VL::get(r, v);
+      }
     }
 #pragma unroll
-    for (int k = 0; k < 24; ++k) S::stn(out + base + k * Ls, o[k]);
+    for (int k = 0; k < 24; ++k) {
+      C t[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) t[v] = o[v][k];
+      VL::stn(out + base + k * Ls, t);
+    }
   }
   if (OUT2) {
     __syncthreads();
     if (valid) {
-      C* u = sh_u + lsite * 12 * Ls + ls;
-      C* l = sh_l + lsite * 12 * Ls + ls;
+      C* u = sh_u + lsite * SU + s0;
+      C* l = sh_l + lsite * SU + s0;
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        u[k * Ls] = o[k] + o[k + 12];
-        l[k * Ls] = o[k] - o[k + 12];
-      }
+      for (int k = 0; k < 12; ++k)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          u[k * Ls + v] = o[v][k] + o[v][k + 12];
+          l[k * Ls + v] = o[v][k] - o[v][k + 12];
+        }
     }
     __syncthreads();
     if (valid) {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) { ou[k] = 0; ol[k] = 0; }
-      const C* u = sh_u + lsite * 12 * Ls;
-      const C* l = sh_l + lsite * 12 * Ls;
+      for (int v = 0; v < VEC; ++v)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) { ou[v][k] = 0; ol[v][k] = 0; }
+      const C* u = sh_u + lsite * SU;
+      const C* l = sh_l + lsite * SU;
       for (int s = 0; s < Ls; ++s) {
-        const C fp = sh_F[s * Ls + ls];
-        const C fm = sh_F[LL + s * Ls + ls];
+        C fp[VEC], fm[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          fp[v] = sh_F[s * Ls + s0 + v];
+          fm[v] = sh_F[LL + s * Ls + s0 + v];
+        }
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-          ou[k] += fp * u[k * Ls + s];
-          ol[k] += fm * l[k * Ls + s];
+          const C us = u[k * Ls + s], ls_ = l[k * Ls + s];
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            ou[v][k] += fp[v] * us;
+            ol[v][k] += fm[v] * ls_;
+          }
         }
       }
       // X2 != 0: out2 = F o + X2 (X2 may be out2 itself: each element is read
@@ -387,14 +486,26 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       if (X2) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-          ou[k] += (C)0.5 * (S::ld(X2 + base + k * Ls) + S::ld(X2 + base + (k + 12) * Ls));
-          ol[k] += (C)0.5 * (S::ld(X2 + base + k * Ls) - S::ld(X2 + base + (k + 12) * Ls));
+          const typename VL::R ra = VL::ld(X2 + base + k * Ls);
+          const typename VL::R rb = VL::ld(X2 + base + (k + 12) * Ls);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            const C a = VL::get(ra, v), b = VL::get(rb, v);
+            ou[v][k] += (C)0.5 * (a + b);
+            ol[v][k] += (C)0.5 * (a - b);
+          }
         }
       }
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
-        S::stn(out2 + base + k * Ls, ou[k] + ol[k]);
-        S::stn(out2 + base + (k + 12) * Ls, ou[k] - ol[k]);
+        C t1[VEC], t2[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          t1[v] = ou[v][k] + ol[v][k];
+          t2[v] = ou[v][k] - ol[v][k];
+        }
+        VL::stn(out2 + base + k * Ls, t1);
+        VL::stn(out2 + base + (k + 12) * Ls, t2);
       }
     }
   }
@@ -405,7 +516,9 @@ static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* ou
                      const void* in, const void* links, const int* nbr, int parity, int Vh,
                      int Ls, int spb, const Epi& e) {
   using C = typename St<T>::C;
-  hipLaunchKernelGGL((k_dwf<T, OP, MAT, ADDX, OUT2, LSC>), grid, block, lds, st, (T*)out,
+  // compile-time Ls (all multiples of 4): vector accesses; runtime Ls: scalar
+  constexpr int VEC = LSC > 0 ? KCfg<T>::VEC : 1;
+  hipLaunchKernelGGL((k_dwf<T, OP, MAT, ADDX, OUT2, LSC, VEC>), grid, block, lds, st, (T*)out,
                      (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
                      (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F,
                      (const C*)e.G, (const T*)e.X2);
@@ -455,7 +568,9 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const bool addx = e.X != nullptr;
   const bool out2 = e.out2 != nullptr;
   if (out2 && !mat) return hipErrorInvalidValue;
-  int spb = DWF_HOP_THREADS / Ls;
+  const int vec = KCfg<T>::VEC;
+  const bool lsc = (Ls == 4 || Ls == 8 || Ls == 12 || Ls == 16) && Ls % vec == 0;
+  int spb = lsc ? KCfg<T>::THREADS / (Ls / vec) : KCfg<T>::THREADS / Ls;
   if (spb < 1) return hipErrorInvalidValue;
   const size_t cs = sizeof(C);
   if ((e.G && (!mat || !addx)) || (e.X2 && !out2)) return hipErrorInvalidValue;
@@ -464,15 +579,12 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const size_t lds_cap = 64 * 1024;
   auto lds_for = [&](int sp) {
     const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * sp * cs;
-    const size_t epi = mat ? 24 * (size_t)sp * Ls * cs : 0;
+    const size_t epi = mat ? 2 * (size_t)sp * (12 * Ls + 1) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
-  const bool full = lds_for(spb) <= lds_cap;
-  while (spb > 1 && lds_for(spb) > lds_cap) --spb;
-  if (lds_for(spb) > lds_cap) return hipErrorInvalidValue;
-  const size_t lds = lds_for(spb);
-  dim3 block(spb * Ls), grid((Vh + spb - 1) / spb);
-  if (full) {
+  if (lsc && lds_for(spb) <= lds_cap) {
+    const size_t lds = lds_for(spb);
+    const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
     switch (Ls) {
       case 4: return launch_combo<T, 4>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
       case 8: return launch_combo<T, 8>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
@@ -481,6 +593,11 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
       default: break;
     }
   }
+  spb = KCfg<T>::THREADS / Ls;
+  while (spb > 1 && lds_for(spb) > lds_cap) --spb;
+  if (lds_for(spb) > lds_cap) return hipErrorInvalidValue;
+  const size_t lds = lds_for(spb);
+  const dim3 block(spb * Ls), grid((Vh + spb - 1) / spb);
   return launch_combo<T, 0>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr,
                             parity, Vh, Ls, spb, e);
 }
