@@ -66,7 +66,17 @@ struct RedScratch {
   unsigned* counter; // 1
   double* result;    // 8
 };
-constexpr int kRedBlocks = 1024;
+#ifndef DWF_RED_BLOCKS
+#define DWF_RED_BLOCKS 1024
+#endif
+constexpr int kRedBlocks = DWF_RED_BLOCKS;
+// Reductions: block partials, then a one-block kernel adds them in a fixed order
+// (1), or the last block of the same kernel does so behind a fence and an atomic
+// counter (0).
+#ifndef DWF_RED_TWO_KERNELS
+#define DWF_RED_TWO_KERNELS 1
+#endif
+constexpr bool kRedTwoKernels = DWF_RED_TWO_KERNELS != 0;
 hipError_t launch_norm2(int prec, const void* x, size_t n, const RedScratch& r, int slot,
                         hipStream_t st);
 hipError_t launch_dot(int prec, const void* x, const void* y, size_t n, const RedScratch& r,

@@ -17,6 +17,11 @@
 #include "dwf_device.h"
 #include "dwf_kernels.h"
 
+// Hop loads of the next direction issued before the current one is computed.
+#ifndef DWF_HOP_PIPE
+#define DWF_HOP_PIPE 0
+#endif
+
 namespace dwf {
 
 size_t prec_bytes(int prec) {
@@ -157,6 +162,79 @@ __device__ __forceinline__ void hop_dir(C (*acc)[24], const T* __restrict__ in, 
   }
 }
 
+// The same split in two halves, so that the loads of the next direction can be
+// in flight while the current one is computed (software pipelining).
+template <int DIR, bool DAG, typename T, int VEC>
+__device__ __forceinline__ void hop_load(typename Vx<T, VEC>::R (&raw)[24], const T* __restrict__ in,
+                                         const int* shN, int lsite, int s0, int spb, int Ls) {
+  using VL = Vx<T, VEC>;
+  const int nb = shN[DIR * spb + lsite];
+  const T* wp = in + (size_t)nb * 24 * Ls + s0;
+  if (!DAG) {
+#pragma unroll
+    for (int k = 0; k < 24; ++k) raw[k] = VL::ld(wp + k * Ls);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      raw[k] = VL::ld(wp + (k + 12) * Ls);
+      raw[k + 12] = VL::ld(wp + k * Ls);
+    }
+  }
+}
+template <int DIR, typename T, int VEC, typename C>
+__device__ __forceinline__ void hop_compute(C (*acc)[24], const typename Vx<T, VEC>::R (&raw)[24],
+                                            const C* shL, int lsite, int spb) {
+  using VL = Vx<T, VEC>;
+  const C* up = shL + DIR * 18 * spb + lsite;
+  C u[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) u[e] = up[e * spb];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    C w[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) w[k] = VL::get(raw[k], v);
+    C h1[6], h2[6];
+    project<DIR>(w, h1, h2);
+    C a1[6], a2[6];
+    su3_mul(u, h1, a1);
+    su3_mul(u, h2, a2);
+    reconstruct<DIR>(acc[v], a1, a2);
+  }
+}
+template <bool DAG, typename T, int VEC, typename C>
+__device__ __forceinline__ void hop_all(C (*acc)[24], const T* __restrict__ in, const C* shL,
+                                        const int* shN, int lsite, int s0, int spb, int Ls) {
+#if DWF_HOP_PIPE
+  typename Vx<T, VEC>::R ra[24], rb[24];
+  hop_load<0, DAG, T, VEC>(ra, in, shN, lsite, s0, spb, Ls);
+  hop_load<1, DAG, T, VEC>(rb, in, shN, lsite, s0, spb, Ls);
+  hop_compute<0, T, VEC>(acc, ra, shL, lsite, spb);
+  hop_load<2, DAG, T, VEC>(ra, in, shN, lsite, s0, spb, Ls);
+  hop_compute<1, T, VEC>(acc, rb, shL, lsite, spb);
+  hop_load<3, DAG, T, VEC>(rb, in, shN, lsite, s0, spb, Ls);
+  hop_compute<2, T, VEC>(acc, ra, shL, lsite, spb);
+  hop_load<4, DAG, T, VEC>(ra, in, shN, lsite, s0, spb, Ls);
+  hop_compute<3, T, VEC>(acc, rb, shL, lsite, spb);
+  hop_load<5, DAG, T, VEC>(rb, in, shN, lsite, s0, spb, Ls);
+  hop_compute<4, T, VEC>(acc, ra, shL, lsite, spb);
+  hop_load<6, DAG, T, VEC>(ra, in, shN, lsite, s0, spb, Ls);
+  hop_compute<5, T, VEC>(acc, rb, shL, lsite, spb);
+  hop_load<7, DAG, T, VEC>(rb, in, shN, lsite, s0, spb, Ls);
+  hop_compute<6, T, VEC>(acc, ra, shL, lsite, spb);
+  hop_compute<7, T, VEC>(acc, rb, shL, lsite, spb);
+#else
+  hop_dir<0, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<1, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<2, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<3, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<4, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<5, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<6, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+  hop_dir<7, DAG, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+#endif
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Remap the hardware workgroup id so that every XCD walks a contiguous
 // range of sites: neighbouring sites then share an L2.
@@ -289,23 +367,9 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         for (int v = 0; v < VEC; ++v) acc[v][k] = VL::get(r, v);
       }
     } else if (OP == K_HOP) {
-      hop_dir<0, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<1, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<2, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<3, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<4, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<5, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<6, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<7, false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_all<false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
     } else {
-      hop_dir<0, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<1, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<2, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<3, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<4, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<5, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<6, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
-      hop_dir<7, true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      hop_all<true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
 #pragma unroll
       for (int v = 0; v < VEC; ++v)
 #pragma unroll
@@ -988,10 +1052,13 @@ __device__ void block_finish(double (&v)[NR], double* partial, unsigned* counter
       for (int w = 0; w < nw; ++w) t += sh[r][w];
       partial[(size_t)blockIdx.x * NR + r] = t;
     }
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    amlast = (prev == gridDim.x - 1);
+    if (!kRedTwoKernels) {
+      __threadfence();
+      unsigned prev = atomicAdd(counter, 1u);
+      amlast = (prev == gridDim.x - 1);
+    }
   }
+  if (kRedTwoKernels) return;   // k_red_finish adds the block partials
   __syncthreads();
   if (!amlast) return;
   __threadfence();
@@ -1020,6 +1087,42 @@ __device__ void block_finish(double (&v)[NR], double* partial, unsigned* counter
     }
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// Second pass of a reduction: block y adds the nblocks partials (NR values
+// each) at partial + y*pstride in a fixed order into result + y*rstride.
+template <int NR>
+__global__ void __launch_bounds__(256)
+k_red_finish(const double* __restrict__ partial, unsigned nblocks, size_t pstride,
+             double* __restrict__ result, int rstride) {
+  const double* p = partial + (size_t)blockIdx.y * pstride;
+  double w[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) w[r] = 0;
+  for (unsigned i = threadIdx.x; i < nblocks; i += blockDim.x)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) w[r] += p[(size_t)i * NR + r];
+  __shared__ double sh[NR][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const double t = wave_sum(w[r]);
+    if (lane == 0) sh[r][wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      result[(size_t)blockIdx.y * rstride + r] = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
+  }
+}
+
+template <int NR>
+static void red_finish(const double* partial, unsigned nblocks, double* result, hipStream_t st,
+                       unsigned ny = 1, size_t pstride = 0, int rstride = 0) {
+  if (!kRedTwoKernels) return;
+  hipLaunchKernelGGL(k_red_finish<NR>, dim3(1, ny), dim3(256), 0, st, partial, nblocks, pstride,
+                     result, rstride);
 }
 
 static dim3 red_grid(size_t n) {
@@ -1175,30 +1278,35 @@ template <typename T>
 static void norm2_t(const void* x, size_t n, const RedScratch& r, int slot, hipStream_t st) {
   hipLaunchKernelGGL(k_norm2<T>, red_grid(n), dim3(256), 0, st, (const T*)x, n, r.partial,
                      r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 template <typename T>
 static void dot_t(const void* x, const void* y, size_t n, const RedScratch& r, int slot,
                   hipStream_t st) {
   hipLaunchKernelGGL(k_dot<T>, red_grid(n), dim3(256), 0, st, (const T*)x, (const T*)y, n,
                      r.partial, r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 template <typename T>
 static void cg_xr_t(void* x, void* rr, const void* p, const void* s, double a, size_t n,
                     const RedScratch& r, int slot, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_xr<T>, red_grid(n), dim3(256), 0, st, (T*)x, (T*)rr, (const T*)p,
                      (const T*)s, (typename St<T>::C)a, n, r.partial, r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 template <typename T>
 static void axpy_dot_t(void* y, double a, const void* x, size_t n, const RedScratch& r, int slot,
                        hipStream_t st) {
   hipLaunchKernelGGL(k_axpy_dot<T>, red_grid(n), dim3(256), 0, st, (T*)y, (typename St<T>::C)a,
                      (const T*)x, n, r.partial, r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 template <typename T>
 static void sub_norm2_t(void* rr, const void* ax, const void* b, size_t n, const RedScratch& r,
                         int slot, hipStream_t st) {
   hipLaunchKernelGGL(k_sub_norm2<T>, red_grid(n), dim3(256), 0, st, (T*)rr, (const T*)ax,
                      (const T*)b, n, r.partial, r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 
 hipError_t launch_norm2(int prec, const void* x, size_t n, const RedScratch& r, int slot,
@@ -1221,6 +1329,7 @@ hipError_t launch_dotc_ls(int prec, const void* x, const void* y, size_t n, int 
     case PBF16: hipLaunchKernelGGL(k_dotc<__hip_bfloat16>, red_grid(np), dim3(256), 0, st, (const __hip_bfloat16*)x, (const __hip_bfloat16*)y, np, Ls, r.partial, r.counter, r.result + slot); break;
     default: return hipErrorInvalidValue;
   }
+  red_finish<2>(r.partial, red_grid(np).x, r.result + slot, st);
   return hipGetLastError();
 }
 hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s, double a,
@@ -1286,6 +1395,7 @@ hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
   g.y = m - 1;
   hipLaunchKernelGGL(k_shift_update, g, dim3(256), 0, st, (double* const*)x, (double* const*)p,
                      (const double*)r, coef, n, partial, counters, pp);
+  red_finish<1>(partial, g.x, pp + 1, st, g.y, kRedBlocks, 1);
   return hipGetLastError();
 }
 
