@@ -58,6 +58,7 @@ struct Ctx {
   void** sh_ptrs = nullptr;       // device array: x[0..n), p[0..n)
   double* host_coef = nullptr;    // pinned
   int sh_cap = 0;
+  hipEvent_t ring_ev[2] = {nullptr, nullptr};   // convergence checks of the sync-free CG
 };
 static Ctx g_ctx;
 static std::mutex g_mtx;
@@ -76,6 +77,7 @@ static int ctx_init() {
   HIPCHK(hipMemset(g_ctx.red.counter, 0, sizeof(unsigned) * 4));
   HIPCHK(hipMalloc(&g_ctx.red.result, sizeof(double) * 16));
   HIPCHK(hipHostMalloc(&g_ctx.host_result, sizeof(double) * 64, hipHostMallocDefault));
+  for (auto& e : g_ctx.ring_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   g_ctx.ok = true;
   return DWF_OK;
 }
@@ -245,6 +247,7 @@ struct dwf_fopr_s {
   void* work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool have_config = false;
   std::vector<ChiMat> hmats;   // host copies of the matrices (long double)
+  double* hop_partial = nullptr;   // block partials of fused norms (Vh doubles)
 };
 
 static size_t cbytes(int prec) { return prec == P64 ? 8 : 4; }
@@ -468,6 +471,33 @@ static int op_DdagD(dwf_fopr_s* op, void* v, const void* w) {
   RC(op_Ddag(op, v, u, u2));
   return DWF_OK;
 }
+// DdagD with |D w|^2 (= <w, DdagD w>) reduced into the device scalar *nrm: the
+// norm is fused into the epilogue of the last hop of D
+static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm) {
+  void *t0, *t1, *u, *u2;
+  RC(get_work(op, 0, &t0));
+  RC(get_work(op, 1, &t1));
+  RC(get_work(op, 2, &u));
+  RC(get_work(op, 3, &u2));
+  if (!op->hop_partial) HIPCHK(hipMalloc(&op->hop_partial, (size_t)op->Vh * sizeof(double)));
+  const void* src = w;
+  if (!op->mp_scalar) {
+    RC(run_site(op, t0, w, M_PRE));
+    src = t0;
+  }
+  RC(run_hop(op, false, 1, t1, src, M_E1));
+  Epi e;
+  e.E = mat(op, M_E2);
+  e.X = w;
+  e.xcoef = 1.0;
+  e.out2 = u2;
+  e.F = mat(op, M_DIT);
+  e.nrm_partial = op->hop_partial;
+  e.nrm_result = nrm;
+  HIPCHK(launch_dwf(op->prec, K_HOP, u, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
+                    g_ctx.stream));
+  return op_Ddag(op, v, u, u2);
+}
 static int op_DDdag(dwf_fopr_s* op, void* v, const void* w) {
   void* u;
   RC(get_work(op, 2, &u));
@@ -630,6 +660,7 @@ int dwf_fopr_destroy(dwf_fopr op) {
   if (op->nbr) (void)hipFree(op->nbr);
   if (op->mats) (void)hipFree(op->mats);
   if (op->stage) (void)hipFree(op->stage);
+  if (op->hop_partial) (void)hipFree(op->hop_partial);
   for (auto& w : op->work)
     if (w) (void)hipFree(w);
   delete op;
@@ -1008,6 +1039,7 @@ struct dwf_solver_s {
   dwf_solver_s* inner = nullptr;
   int inner_iters = 0;
   dwf_lpc_s* lpc = nullptr;
+  bool fast = false;   // inner CG: the sync-free iteration (cg_solve_fast)
 };
 
 static int solver_field(dwf_fopr op, void** slot) {
@@ -1075,6 +1107,71 @@ static int cg_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out, do
   *diff_out = std::sqrt(diff2 * snorm);
   s->inner_iters = nconv > 0 ? nconv : s->niter;
   return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "ASolver_CG: not converged") : DWF_OK;
+}
+
+// CG on op's DdagD from a zero guess without host round trips inside the
+// iteration: <p, A p> = |D p|^2 comes from the epilogue of the last hop of D,
+// the coefficients stay in device memory (ping-pong slots), and |r|^2 of
+// iteration i is read back while iteration i + 1 already runs.  The
+// iteration that passes the test is followed by one more (never fewer
+// iterations than the test asks for).  Used as the inner solver of the
+// mixed-precision Richardson, where only the result matters.
+static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
+                         double* diff_out) {
+  dwf_fopr op = s->op;
+  for (int i = 0; i < 4; ++i) RC(solver_field(op, &s->f[i]));
+  void *x = s->f[0], *r = s->f[1], *p = s->f[2], *q = s->f[3];
+  const int pr = op->prec;
+  const size_t n = op->nelem;
+  hipStream_t st = g_ctx.stream;
+  double* res = g_ctx.red.result;
+  const int RA = 8, RB = 9, PAP = 10;
+  double sr;
+  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &sr));
+  if (!(sr > 0)) {
+    HIPCHK(launch_set_zero(pr, xq, n, st));
+    *nconv_out = 0;
+    *diff_out = 0;
+    s->inner_iters = 0;
+    return DWF_OK;
+  }
+  const double snorm = 1.0 / sr;
+  HIPCHK(launch_copy(pr, r, b, n, st));
+  HIPCHK(launch_copy(pr, p, b, n, st));
+  HIPCHK(launch_set_zero(pr, x, n, st));
+  HIPCHK(launch_norm2(pr, r, n, g_ctx.red, RA, st));
+  double* ring = g_ctx.host_result + 32;
+  int cur = RA, nxt = RB, nconv = -1, iter = 0;
+  for (; iter < s->niter; ++iter) {
+    RC(op_DdagD_nrm(op, q, p, res + PAP));
+    HIPCHK(launch_cg_xr_dev(pr, x, r, p, q, res + cur, res + PAP, n, g_ctx.red, nxt, st));
+    HIPCHK(launch_aypx_dev(pr, p, r, res + nxt, res + cur, n, st));
+    HIPCHK(hipMemcpyAsync(ring + (iter & 1), res + nxt, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(g_ctx.ring_ev[iter & 1], st));
+    std::swap(cur, nxt);
+    if (iter >= 1) {
+      HIPCHK(hipEventSynchronize(g_ctx.ring_ev[(iter - 1) & 1]));
+      if (ring[(iter - 1) & 1] * snorm < s->stop) {
+        nconv = iter;   // the test passed after iteration iter - 1 (0-based)
+        ++iter;         // iterations run: iter + 1
+        break;
+      }
+    }
+  }
+  if (nconv < 0 && iter == s->niter && iter >= 1) {   // the last iteration's test
+    HIPCHK(hipEventSynchronize(g_ctx.ring_ev[(iter - 1) & 1]));
+    if (ring[(iter - 1) & 1] * snorm < s->stop) nconv = iter;
+  }
+  HIPCHK(launch_copy(pr, xq, x, n, st));
+  RC(mult_impl(op, q, xq, op->mode, false));
+  double diff2;
+  HIPCHK(launch_sub_norm2(pr, q, q, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &diff2));
+  *nconv_out = nconv;
+  *diff_out = std::sqrt(diff2 * snorm);
+  s->inner_iters = iter;
+  return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "CG: not converged") : DWF_OK;
 }
 
 // Flexible preconditioned CG on A = op (mode DdagD) with M = K K^+, zero
@@ -1193,7 +1290,9 @@ static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv
     int inc = 0;
     double idiff = 0;
     int rc = s->inner->kind == 2 ? fcg_solve(s->inner, sf, rf, &inc, &idiff)
-                                 : cg_solve(s->inner, sf, rf, &inc, &idiff);
+             : (s->inner->fast && opf->mode == "DdagD")
+                 ? cg_solve_fast(s->inner, sf, rf, &inc, &idiff)
+                 : cg_solve(s->inner, sf, rf, &inc, &idiff);
     if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
     s->inner_iters += s->inner->inner_iters;
     HIPCHK(launch_convert(pr, sv, opf->prec, sf, n, st));
@@ -1246,6 +1345,7 @@ int dwf_solver_create_richardson(dwf_fopr op_d, dwf_fopr op_f, int max_iter, dou
   s->inner->niter = max_iter_inner;
   s->inner->stop = stop_cond_inner;
   s->inner->guess = DWF_GUESS_ZERO;
+  s->inner->fast = true;
   *out = s;
   return DWF_OK;
 }

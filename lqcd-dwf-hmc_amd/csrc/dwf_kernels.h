@@ -36,6 +36,8 @@ struct Epi {
   const void* F = nullptr;
   const void* G = nullptr;    // with X: out = E core + G X (chiral matrix) instead of + xcoef X
   const void* X2 = nullptr;   // with out2: out2 = F out + X2 (X2 may alias out2)
+  double* nrm_partial = nullptr;   // with E: |out|^2 -> nrm_result (block partials, >= Vh doubles)
+  double* nrm_result = nullptr;
 };
 
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
@@ -93,6 +95,14 @@ hipError_t launch_axpy_dot(int prec, void* y, double a, const void* x, size_t n,
 // r = (A x) - b, result[slot] = |r|^2   (r may alias Ax)
 hipError_t launch_sub_norm2(int prec, void* r, const void* ax, const void* b, size_t n,
                             const RedScratch& red, int slot, hipStream_t st);
+
+// CG updates with device-resident coefficients a = num[0] / den[0]:
+//   x += a p, r -= a s, |r|^2 -> result[slot];   y = a y + x
+hipError_t launch_cg_xr_dev(int prec, void* x, void* r, const void* p, const void* s,
+                            const double* num, const double* den, size_t n,
+                            const RedScratch& red, int slot, hipStream_t st);
+hipError_t launch_aypx_dev(int prec, void* y, const void* x, const double* num, const double* den,
+                           size_t n, hipStream_t st);
 
 // Multishift update of active shifts j = 1..m-1 (arrays of device pointers on
 // the device; coefficients on the device, 3 per shift: -beta_s, alpha_s, zeta):

@@ -24,6 +24,12 @@
 
 namespace dwf {
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
 size_t prec_bytes(int prec) {
   switch (prec) {
     case P64: return 8;
@@ -244,6 +250,47 @@ __device__ __forceinline__ int xcd_block(int w, int G) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Matrix-core tile of the chiral epilogue: D(16x16) += A(16x4) B(4x16), lane l
+// holding A[l % 16][l / 16] and B[l / 16][l % 16].  Register r of lane l holds
+// D[4 (l / 16) + r][l % 16] for f32 and D[l / 16 + 4 r][l % 16] for f64 (both
+// measured on gfx950).  row(i) names the output s of MFMA row i so that in
+// both cases lane l ends up with s = 4 (l / 16) + r: identity for f32, the
+// interleave 4 (i % 4) + i / 4 for f64.
+template <typename C> struct Mma;
+template <> struct Mma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 run(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int i) { return i; }
+};
+template <> struct Mma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 run(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int i) { return 4 * (i & 3) + (i >> 2); }
+};
+// four consecutive elements of a field (s_out = 4q .. 4q + 3 of one row)
+template <typename T> struct Q4 {
+  static __device__ __forceinline__ void ld(const T* p, float* o) {
+    const typename Vx<T, 4>::R r = Vx<T, 4>::ld(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = Vx<T, 4>::get(r, i);
+  }
+  static __device__ __forceinline__ void stn(T* p, const float* v) { Vx<T, 4>::stn(p, v); }
+};
+template <> struct Q4<double> {
+  static __device__ __forceinline__ void ld(const double* p, double* o) {
+    const dbl2_t a = Vx<double, 2>::ld(p), b = Vx<double, 2>::ld(p + 2);
+    o[0] = a[0]; o[1] = a[1]; o[2] = b[0]; o[3] = b[1];
+  }
+  static __device__ __forceinline__ void stn(double* p, const double* v) {
+    Vx<double, 2>::stn(p, v);
+    Vx<double, 2>::stn(p + 2, v + 2);
+  }
+};
+
 // Occupancy target (waves per SIMD) of the fused kernel per storage type; the
 // compiler caps registers to reach it.
 #ifndef DWF_WAVES_F64
@@ -292,7 +339,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
       const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
       const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
-      const T* X2) {
+      const T* X2, double* __restrict__ nrm) {
   using S = St<T>;
   using C = typename S::C;
   using VL = Vx<T, VEC>;
@@ -305,7 +352,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   // their rows from different LDS banks
   const int SU = 12 * Ls + 1;
   const int stageC = (OP == K_SITE) ? 0 : (144 * spb + 8 * spb);
-  const int epiC = MAT ? 2 * spb * SU : 0;
+  const int epiC = MAT ? (LSC == 16 ? 2 * spb * 12 * 17 : 2 * spb * SU) : 0;
   const int regionA = stageC > epiC ? stageC : epiC;
   C* shA = reinterpret_cast<C*>(smem_raw);
   C* shL = shA;
@@ -400,6 +447,162 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   }
 
   if (OP != K_SITE) __syncthreads();  // staging area is reused below
+  if constexpr (LSC == 16) {
+    // Matrix-core epilogue.  Per chirality the block's core is a 16 x (12 spb)
+    // matrix U[s][col], col = lsite*12 + k; O = E U on 16-column tiles, one tile
+    // pair (P+, P-) per wave at a time.  LDS column stride 17 (bank spread).
+    using MM = Mma<C>;
+    using v4 = typename MM::v4;
+    constexpr int LP = 17;
+    constexpr int NW = KCfg<T>::THREADS / 64;      // waves per block
+    constexpr int NTW = (12 + NW - 1) / NW;        // tiles per wave (12 tiles)
+    C* mu = shA;
+    C* ml = shA + spb * 12 * LP;
+    auto stage_core = [&](const C (*a)[24]) {
+      if (valid) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            mu[(lsite * 12 + k) * LP + s0 + v] = a[v][k] + a[v][k + 12];
+            ml[(lsite * 12 + k) * LP + s0 + v] = a[v][k] - a[v][k + 12];
+          }
+      }
+    };
+    stage_core(acc);
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ntiles = (spb * 12) / 16;
+    v4 cu[NTW], cl[NTW];
+    const int arow = MM::row(li);   // output s of this lane's A row
+    auto mma_pass = [&](const C* M, bool add) {
+#pragma unroll
+      for (int j = 0; j < NTW; ++j) {
+        const int t = wave + j * nwaves;
+        if (t < ntiles) {
+          const int col = t * 16 + li;
+          v4 au = add ? cu[j] : v4{0, 0, 0, 0}, al = add ? cl[j] : v4{0, 0, 0, 0};
+#pragma unroll
+          for (int kc = 0; kc < 4; ++kc) {
+            const int sin = kc * 4 + lk;
+            au = MM::run(M[sin * 16 + arow], mu[col * LP + sin], au);
+            al = MM::run(M[256 + sin * 16 + arow], ml[col * LP + sin], al);
+          }
+          cu[j] = au;
+          cl[j] = al;
+        }
+      }
+    };
+    mma_pass(sh_E, false);
+    if (xmat) {   // + G X: stage X's chiral components in the same buffers
+      __syncthreads();
+      if (valid) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          const typename VL::R ra = VL::ld(X + base + k * Ls);
+          const typename VL::R rb = VL::ld(X + base + (k + 12) * Ls);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            const C a = VL::get(ra, v), b = VL::get(rb, v);
+            mu[(lsite * 12 + k) * LP + s0 + v] = a + b;
+            ml[(lsite * 12 + k) * LP + s0 + v] = a - b;
+          }
+        }
+      }
+      __syncthreads();
+      mma_pass(sh_G, true);
+    }
+    // combine, add X, store; lane: column col, rows s_out = 4 lk + r.  With
+    // OUT2 the result is staged again for F, after every wave's reads.
+    if (OUT2) __syncthreads();
+    double nacc = 0;
+#pragma unroll
+    for (int j = 0; j < NTW; ++j) {
+      const int t = wave + j * nwaves;
+      if (t < ntiles) {
+        const int col = t * 16 + li;
+        const int cs = col / 12, ck = col - cs * 12;
+        const bool cvalid = site0 + cs < Vh;
+        const size_t ob = ((size_t)(site0 + cs) * 24 + ck) * Ls + 4 * lk;
+        C o1[4], o2[4];   // rows k and k + 12
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          o1[r] = cu[j][r] + cl[j][r];
+          o2[r] = cu[j][r] - cl[j][r];
+        }
+        if (cvalid) {
+          if (ADDX && !xmat) {
+            C xa[4], xb[4];
+            Q4<T>::ld(X + ob, xa);
+            Q4<T>::ld(X + ob + 12 * Ls, xb);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              o1[r] += xcoef * xa[r];
+              o2[r] += xcoef * xb[r];
+            }
+          }
+          Q4<T>::stn(out + ob, o1);
+          Q4<T>::stn(out + ob + 12 * Ls, o2);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            nacc += (double)o1[r] * (double)o1[r] + (double)o2[r] * (double)o2[r];
+        }
+        if (OUT2) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            mu[col * LP + 4 * lk + r] = o1[r] + o2[r];
+            ml[col * LP + 4 * lk + r] = o1[r] - o2[r];
+          }
+        }
+      }
+    }
+    if (nrm) {   // |out|^2: one partial per block (uniform branch)
+      __shared__ double wsum2[8];
+      const double t = wave_sum(nacc);
+      if (lane == 0) wsum2[wave] = t;
+      __syncthreads();
+      if (tid == 0) {
+        double a = 0;
+        for (int w = 0; w < nwaves; ++w) a += wsum2[w];
+        nrm[blockIdx.x] = a;
+      }
+    }
+    if (OUT2) {
+      __syncthreads();
+      mma_pass(sh_F, false);
+#pragma unroll
+      for (int j = 0; j < NTW; ++j) {
+        const int t = wave + j * nwaves;
+        if (t < ntiles) {
+          const int col = t * 16 + li;
+          const int cs = col / 12, ck = col - cs * 12;
+          if (site0 + cs < Vh) {
+            const size_t ob = ((size_t)(site0 + cs) * 24 + ck) * Ls + 4 * lk;
+            C p1[4], p2[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              p1[r] = cu[j][r] + cl[j][r];
+              p2[r] = cu[j][r] - cl[j][r];
+            }
+            if (X2) {
+              C xa[4], xb[4];
+              Q4<T>::ld(X2 + ob, xa);
+              Q4<T>::ld(X2 + ob + 12 * Ls, xb);
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                p1[r] += xa[r];
+                p2[r] += xb[r];
+              }
+            }
+            Q4<T>::stn(out2 + ob, p1);
+            Q4<T>::stn(out2 + ob + 12 * Ls, p2);
+          }
+        }
+      }
+    }
+    return;
+  }
   // chiral components of the core, per (site, k < 12, s): P+ ~ u, P- ~ l
   if (valid) {
     C* u = sh_u + lsite * SU + s0;
@@ -507,6 +710,24 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       VL::stn(out + base + k * Ls, t);
     }
   }
+  if (nrm) {   // |out|^2: one partial per block (uniform branch)
+    double t = 0;
+    if (valid) {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v)
+#pragma unroll
+        for (int k = 0; k < 24; ++k) t += (double)o[v][k] * (double)o[v][k];
+    }
+    __shared__ double wsum[8];
+    t = wave_sum(t);
+    if ((tid & 63) == 0) wsum[tid >> 6] = t;
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0;
+      for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) a += wsum[w];
+      nrm[blockIdx.x] = a;
+    }
+  }
   if (OUT2) {
     __syncthreads();
     if (valid) {
@@ -585,7 +806,7 @@ static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* ou
   hipLaunchKernelGGL((k_dwf<T, OP, MAT, ADDX, OUT2, LSC, VEC>), grid, block, lds, st, (T*)out,
                      (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
                      (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F,
-                     (const C*)e.G, (const T*)e.X2);
+                     (const C*)e.G, (const T*)e.X2, e.nrm_partial);
 }
 
 // the (op, epilogue) combinations the operator chains use
@@ -623,15 +844,18 @@ static hipError_t launch_combo(int op, bool mat, bool addx, bool out2, dim3 g, d
   return hipGetLastError();
 }
 
+static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st);
+
 template <typename T>
 static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* links,
                                const int* nbr, int parity, int Vh, int Ls, const Epi& e,
-                               hipStream_t st) {
+                               hipStream_t st, unsigned* nblocks) {
   using C = typename St<T>::C;
   const bool mat = e.E != nullptr;
   const bool addx = e.X != nullptr;
   const bool out2 = e.out2 != nullptr;
   if (out2 && !mat) return hipErrorInvalidValue;
+  if (e.nrm_partial && !mat) return hipErrorInvalidValue;
   const int vec = KCfg<T>::VEC;
   const bool lsc = (Ls == 4 || Ls == 8 || Ls == 12 || Ls == 16) && Ls % vec == 0;
   int spb = lsc ? KCfg<T>::THREADS / (Ls / vec) : KCfg<T>::THREADS / Ls;
@@ -643,12 +867,13 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const size_t lds_cap = 64 * 1024;
   auto lds_for = [&](int sp) {
     const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * sp * cs;
-    const size_t epi = mat ? 2 * (size_t)sp * (12 * Ls + 1) * cs : 0;
+    const size_t epi = mat ? 2 * (size_t)sp * (Ls == 16 ? 12 * 17 : 12 * Ls + 1) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
     const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
+    *nblocks = grid.x;
     switch (Ls) {
       case 4: return launch_combo<T, 4>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
       case 8: return launch_combo<T, 8>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
@@ -662,6 +887,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   if (lds_for(spb) > lds_cap) return hipErrorInvalidValue;
   const size_t lds = lds_for(spb);
   const dim3 block(spb * Ls), grid((Vh + spb - 1) / spb);
+  *nblocks = grid.x;
   return launch_combo<T, 0>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr,
                             parity, Vh, Ls, spb, e);
 }
@@ -669,14 +895,21 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
                       const int* nbr, int parity, int Vh, int Ls, const Epi& e,
                       hipStream_t st) {
+  hipError_t err = hipErrorInvalidValue;
+  unsigned nb = 0;
   switch (prec) {
-    case P64: return launch_dwf_t<double>(op, out, in, links, nbr, parity, Vh, Ls, e, st);
-    case P32: return launch_dwf_t<float>(op, out, in, links, nbr, parity, Vh, Ls, e, st);
-    case P16: return launch_dwf_t<__half>(op, out, in, links, nbr, parity, Vh, Ls, e, st);
+    case P64: err = launch_dwf_t<double>(op, out, in, links, nbr, parity, Vh, Ls, e, st, &nb); break;
+    case P32: err = launch_dwf_t<float>(op, out, in, links, nbr, parity, Vh, Ls, e, st, &nb); break;
+    case P16: err = launch_dwf_t<__half>(op, out, in, links, nbr, parity, Vh, Ls, e, st, &nb); break;
     case PBF16:
-      return launch_dwf_t<__hip_bfloat16>(op, out, in, links, nbr, parity, Vh, Ls, e, st);
+      err = launch_dwf_t<__hip_bfloat16>(op, out, in, links, nbr, parity, Vh, Ls, e, st, &nb);
+      break;
   }
-  return hipErrorInvalidValue;
+  if (err == hipSuccess && e.nrm_partial) {
+    finish_norm(e.nrm_partial, nb, e.nrm_result, st);
+    err = hipGetLastError();
+  }
+  return err;
 }
 
 // ---------------------------------------------------------------------------
@@ -1026,11 +1259,6 @@ hipError_t launch_convert(int prec_out, void* y, int prec_in, const void* x, siz
 // reductions: block partials, then the last block to finish sums the partials
 // in a fixed order (deterministic) and resets the counter.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
 
 template <int NR>
 __device__ void block_finish(double (&v)[NR], double* partial, unsigned* counter,
@@ -1123,6 +1351,11 @@ static void red_finish(const double* partial, unsigned nblocks, double* result, 
   if (!kRedTwoKernels) return;
   hipLaunchKernelGGL(k_red_finish<NR>, dim3(1, ny), dim3(256), 0, st, partial, nblocks, pstride,
                      result, rstride);
+}
+
+static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st) {
+  hipLaunchKernelGGL(k_red_finish<1>, dim3(1, 1), dim3(256), 0, st, partial, nblocks, (size_t)0,
+                     result, 0);
 }
 
 static dim3 red_grid(size_t n) {
@@ -1335,6 +1568,82 @@ hipError_t launch_dotc_ls(int prec, const void* x, const void* y, size_t n, int 
 hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s, double a,
                         size_t n, const RedScratch& red, int slot, hipStream_t st) {
   DWF_DISPATCH(prec, cg_xr_t, (x, r, p, s, a, n, red, slot, st));
+  return hipGetLastError();
+}
+
+// CG updates with the coefficients in device memory (no host round trip):
+//   x += a p, r -= a s, |r|^2 -> result[slot], a = num[0] / den[0]
+//   p = (num[0] / den[0]) p + r
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_cg_xr_dev(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+            const T* __restrict__ s, const double* num, const double* den, size_t n,
+            double* partial, unsigned* counter, double* result) {
+  using C = typename St<T>::C;
+  const C a = (C)(num[0] / den[0]);
+  double v[1] = {0};
+  PACK_LOOP(T, n) {
+    C xv[NV], rv[NV], pv[NV], sv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(p + ip * NV, pv);
+    Vec<T>::ld(r + ip * NV, rv);
+    Vec<T>::ld(s + ip * NV, sv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      xv[j] += a * pv[j];
+      rv[j] = rounded<T>(rv[j] - a * sv[j]);
+      v[0] += (double)rv[j] * (double)rv[j];
+    }
+    Vec<T>::st(x + ip * NV, xv);
+    Vec<T>::st(r + ip * NV, rv);
+  }
+  TAIL_LOOP(n) {
+    St<T>::st(x + i, St<T>::ld(x + i) + a * St<T>::ld(p + i));
+    C rn = rounded<T>(St<T>::ld(r + i) - a * St<T>::ld(s + i));
+    St<T>::st(r + i, rn);
+    v[0] += (double)rn * (double)rn;
+  }
+  block_finish<1>(v, partial, counter, result);
+}
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_aypx_dev(T* __restrict__ y, const T* __restrict__ x, const double* num, const double* den,
+           size_t n) {
+  using C = typename St<T>::C;
+  const C a = (C)(num[0] / den[0]);
+  PACK_LOOP(T, n) {
+    C yv[NV], xv[NV];
+    Vec<T>::ld(y + ip * NV, yv);
+    Vec<T>::ld(x + ip * NV, xv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) yv[j] = a * yv[j] + xv[j];
+    Vec<T>::st(y + ip * NV, yv);
+  }
+  TAIL_LOOP(n) St<T>::st(y + i, a * St<T>::ld(y + i) + St<T>::ld(x + i));
+}
+template <typename T>
+static void cg_xr_dev_t(void* x, void* rr, const void* p, const void* s, const double* num,
+                        const double* den, size_t n, const RedScratch& r, int slot,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_xr_dev<T>, red_grid(n), dim3(256), 0, st, (T*)x, (T*)rr, (const T*)p,
+                     (const T*)s, num, den, n, r.partial, r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
+}
+template <typename T>
+static void aypx_dev_t(void* y, const void* x, const double* num, const double* den, size_t n,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_aypx_dev<T>, grid_for(n), dim3(256), 0, st, (T*)y, (const T*)x, num, den,
+                     n);
+}
+hipError_t launch_cg_xr_dev(int prec, void* x, void* r, const void* p, const void* s,
+                            const double* num, const double* den, size_t n,
+                            const RedScratch& red, int slot, hipStream_t st) {
+  DWF_DISPATCH(prec, cg_xr_dev_t, (x, r, p, s, num, den, n, red, slot, st));
+  return hipGetLastError();
+}
+hipError_t launch_aypx_dev(int prec, void* y, const void* x, const double* num, const double* den,
+                           size_t n, hipStream_t st) {
+  DWF_DISPATCH(prec, aypx_dev_t, (y, x, num, den, n, st));
   return hipGetLastError();
 }
 hipError_t launch_axpy_dot(int prec, void* y, double a, const void* x, size_t n,

@@ -21,6 +21,8 @@ PARAMS = [
     ((4, 4, 2, 2), 5, 0.3, 1.3, 1.2, 0.3, (-1, 1, -1, -1)),
     ((8, 8, 8, 8), 8, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1)),
 ]
+LS16 = [((4, 4, 4, 8), 16, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1)),      # matrix-core epilogue
+        ((4, 2, 4, 4), 16, 0.05, 1.0, 1.5, 0.5, (1, 1, 1, -1))]
 MODES = ["D", "Ddag", "DdagD", "DDdag", "H", "Hdag", "Deo", "Doe", "Dee", "Doo",
          "Dee_inv", "Doo_inv"]
 
@@ -39,7 +41,7 @@ def pair(prm, precision="double", seed=5, eps=0.5):
     return orc, gpu, U
 
 
-@pytest.mark.parametrize("prm", PARAMS)
+@pytest.mark.parametrize("prm", PARAMS + LS16[1:])
 def test_modes_fp64(prm):
     print("case", prm, flush=True)
     orc, gpu, U = pair(prm)
@@ -51,7 +53,7 @@ def test_modes_fp64(prm):
             assert rel(got, want) < 1e-13, (mode, dag, rel(got, want))
 
 
-@pytest.mark.parametrize("prm", PARAMS[:2])
+@pytest.mark.parametrize("prm", PARAMS[:2] + LS16)
 @pytest.mark.parametrize("precision,tol", [("float", 2e-6), ("half", 4e-3), ("bfloat16", 3e-2)])
 def test_modes_reduced_precision(prm, precision, tol):
     orc, gpu, U = pair(prm, precision)
