@@ -112,6 +112,31 @@ __device__ __forceinline__ void reconstruct(C* o, const C* a1, const C* a2) {
   }
 }
 
+// a1 = U h1, a2 = U h2 with U read from LDS element by element (row-major
+// complex, entry ab = ri + 2*(b + 3*a) at up[ab * stride]): each entry is used
+// for both half spinors and dies right after, so the link never occupies 18
+// registers.
+template <typename C>
+__device__ __forceinline__ void su3_mul2_lds(const C* up, int stride, const C* h1, const C* h2,
+                                             C* a1, C* a2) {
+#pragma unroll
+  for (int ia = 0; ia < 3; ++ia) {
+    C r1 = 0, i1 = 0, r2 = 0, i2 = 0;
+#pragma unroll
+    for (int ib = 0; ib < 3; ++ib) {
+      const C ur = up[(2 * (ib + 3 * ia)) * stride], ui = up[(2 * (ib + 3 * ia) + 1) * stride];
+      r1 += ur * h1[2 * ib] - ui * h1[2 * ib + 1];
+      i1 += ur * h1[2 * ib + 1] + ui * h1[2 * ib];
+      r2 += ur * h2[2 * ib] - ui * h2[2 * ib + 1];
+      i2 += ur * h2[2 * ib + 1] + ui * h2[2 * ib];
+    }
+    a1[2 * ia] = r1;
+    a1[2 * ia + 1] = i1;
+    a2[2 * ia] = r2;
+    a2[2 * ia + 1] = i2;
+  }
+}
+
 // a = U h for a 3x3 row-major complex matrix u[ab], ab = ri + 2*(b + 3*a)
 template <typename C>
 __device__ __forceinline__ void su3_mul(const C* u, const C* h, C* a) {
@@ -151,9 +176,6 @@ __device__ __forceinline__ void hop_dir(C (*acc)[24], const T* __restrict__ in, 
     }
   }
   const C* up = shL + DIR * 18 * spb + lsite;
-  C u[18];
-#pragma unroll
-  for (int e = 0; e < 18; ++e) u[e] = up[e * spb];
 #pragma unroll
   for (int v = 0; v < VEC; ++v) {
     C w[24];
@@ -162,8 +184,7 @@ __device__ __forceinline__ void hop_dir(C (*acc)[24], const T* __restrict__ in, 
     C h1[6], h2[6];
     project<DIR>(w, h1, h2);
     C a1[6], a2[6];
-    su3_mul(u, h1, a1);
-    su3_mul(u, h2, a2);
+    su3_mul2_lds(up, spb, h1, h2, a1, a2);
     reconstruct<DIR>(acc[v], a1, a2);
   }
 }
@@ -294,7 +315,7 @@ template <> struct Q4<double> {
 // Occupancy target (waves per SIMD) of the fused kernel per storage type; the
 // compiler caps registers to reach it.
 #ifndef DWF_WAVES_F64
-#define DWF_WAVES_F64 3
+#define DWF_WAVES_F64 4
 #endif
 #ifndef DWF_WAVES_F32
 #define DWF_WAVES_F32 4
@@ -352,7 +373,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   // their rows from different LDS banks
   const int SU = 12 * Ls + 1;
   const int stageC = (OP == K_SITE) ? 0 : (144 * spb + 8 * spb);
-  const int epiC = MAT ? (LSC == 16 ? 2 * spb * 12 * 17 : 2 * spb * SU) : 0;
+  const int epiC = MAT ? (LSC == 16 ? spb * 12 * 17 : 2 * spb * SU) : 0;
   const int regionA = stageC > epiC ? stageC : epiC;
   C* shA = reinterpret_cast<C*>(smem_raw);
   C* shL = shA;
@@ -449,73 +470,79 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   if (OP != K_SITE) __syncthreads();  // staging area is reused below
   if constexpr (LSC == 16) {
     // Matrix-core epilogue.  Per chirality the block's core is a 16 x (12 spb)
-    // matrix U[s][col], col = lsite*12 + k; O = E U on 16-column tiles, one tile
-    // pair (P+, P-) per wave at a time.  LDS column stride 17 (bank spread).
+    // matrix U[s][col], col = lsite*12 + k; O = E U on 16-column tiles.  One
+    // chirality at a time goes through one LDS buffer (column stride 17, bank
+    // spread), which halves the LDS per block.
     using MM = Mma<C>;
     using v4 = typename MM::v4;
     constexpr int LP = 17;
     constexpr int NW = KCfg<T>::THREADS / 64;      // waves per block
     constexpr int NTW = (12 + NW - 1) / NW;        // tiles per wave (12 tiles)
-    C* mu = shA;
-    C* ml = shA + spb * 12 * LP;
-    auto stage_core = [&](const C (*a)[24]) {
-      if (valid) {
-#pragma unroll
-        for (int k = 0; k < 12; ++k)
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            mu[(lsite * 12 + k) * LP + s0 + v] = a[v][k] + a[v][k + 12];
-            ml[(lsite * 12 + k) * LP + s0 + v] = a[v][k] - a[v][k + 12];
-          }
-      }
-    };
-    stage_core(acc);
-    __syncthreads();
+    C* mb = shA;
     const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int ntiles = (spb * 12) / 16;
-    v4 cu[NTW], cl[NTW];
     const int arow = MM::row(li);   // output s of this lane's A row
-    auto mma_pass = [&](const C* M, bool add) {
+    v4 cu[NTW], cl[NTW];
+    // D[j] (+)= M_chi B on the staged chirality
+    auto mma_one = [&](const C* M, v4 (&D)[NTW], bool add) {
 #pragma unroll
       for (int j = 0; j < NTW; ++j) {
         const int t = wave + j * nwaves;
         if (t < ntiles) {
           const int col = t * 16 + li;
-          v4 au = add ? cu[j] : v4{0, 0, 0, 0}, al = add ? cl[j] : v4{0, 0, 0, 0};
+          v4 a = add ? D[j] : v4{0, 0, 0, 0};
 #pragma unroll
           for (int kc = 0; kc < 4; ++kc) {
             const int sin = kc * 4 + lk;
-            au = MM::run(M[sin * 16 + arow], mu[col * LP + sin], au);
-            al = MM::run(M[256 + sin * 16 + arow], ml[col * LP + sin], al);
+            a = MM::run(M[sin * 16 + arow], mb[col * LP + sin], a);
           }
-          cu[j] = au;
-          cl[j] = al;
+          D[j] = a;
         }
       }
     };
-    mma_pass(sh_E, false);
-    if (xmat) {   // + G X: stage X's chiral components in the same buffers
-      __syncthreads();
+    // core: u = acc_k + acc_{k+12} (P+), then l = acc_k - acc_{k+12} (P-)
+    auto stage_acc = [&](C sign) {
       if (valid) {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          const typename VL::R ra = VL::ld(X + base + k * Ls);
-          const typename VL::R rb = VL::ld(X + base + (k + 12) * Ls);
+        for (int k = 0; k < 12; ++k)
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            const C a = VL::get(ra, v), b = VL::get(rb, v);
-            mu[(lsite * 12 + k) * LP + s0 + v] = a + b;
-            ml[(lsite * 12 + k) * LP + s0 + v] = a - b;
+          for (int v = 0; v < VEC; ++v)
+            mb[(lsite * 12 + k) * LP + s0 + v] = acc[v][k] + sign * acc[v][k + 12];
+      }
+    };
+    stage_acc(C(1));
+    __syncthreads();
+    mma_one(sh_E, cu, false);
+    __syncthreads();
+    stage_acc(C(-1));
+    __syncthreads();
+    mma_one(sh_E + 256, cl, false);
+    if (xmat) {   // + G X, X staged the same way
+      auto stage_x = [&](C sign) {
+        if (valid) {
+#pragma unroll
+          for (int k = 0; k < 12; ++k) {
+            const typename VL::R ra = VL::ld(X + base + k * Ls);
+            const typename VL::R rb = VL::ld(X + base + (k + 12) * Ls);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+              mb[(lsite * 12 + k) * LP + s0 + v] = VL::get(ra, v) + sign * VL::get(rb, v);
           }
         }
-      }
+      };
       __syncthreads();
-      mma_pass(sh_G, true);
+      stage_x(C(1));
+      __syncthreads();
+      mma_one(sh_G, cu, true);
+      __syncthreads();
+      stage_x(C(-1));
+      __syncthreads();
+      mma_one(sh_G + 256, cl, true);
     }
     // combine, add X, store; lane: column col, rows s_out = 4 lk + r.  With
-    // OUT2 the result is staged again for F, after every wave's reads.
-    if (OUT2) __syncthreads();
+    // OUT2 the result is kept in cu/cl (as o_k + o_{k+12}, o_k - o_{k+12}) and
+    // staged again for F.
     double nacc = 0;
 #pragma unroll
     for (int j = 0; j < NTW; ++j) {
@@ -551,8 +578,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         if (OUT2) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            mu[col * LP + 4 * lk + r] = o1[r] + o2[r];
-            ml[col * LP + 4 * lk + r] = o1[r] - o2[r];
+            cu[j][r] = o1[r] + o2[r];
+            cl[j][r] = o1[r] - o2[r];
           }
         }
       }
@@ -569,8 +596,26 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       }
     }
     if (OUT2) {
+      v4 fu[NTW], fl[NTW];
+      auto stage_o = [&](const v4 (&src)[NTW]) {
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+          const int t = wave + j * nwaves;
+          if (t < ntiles) {
+            const int col = t * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = src[j][r];
+          }
+        }
+      };
       __syncthreads();
-      mma_pass(sh_F, false);
+      stage_o(cu);
+      __syncthreads();
+      mma_one(sh_F, fu, false);
+      __syncthreads();
+      stage_o(cl);
+      __syncthreads();
+      mma_one(sh_F + 256, fl, false);
 #pragma unroll
       for (int j = 0; j < NTW; ++j) {
         const int t = wave + j * nwaves;
@@ -582,8 +627,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             C p1[4], p2[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              p1[r] = cu[j][r] + cl[j][r];
-              p2[r] = cu[j][r] - cl[j][r];
+              p1[r] = fu[j][r] + fl[j][r];
+              p2[r] = fu[j][r] - fl[j][r];
             }
             if (X2) {
               C xa[4], xb[4];
@@ -867,7 +912,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const size_t lds_cap = 64 * 1024;
   auto lds_for = [&](int sp) {
     const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * sp * cs;
-    const size_t epi = mat ? 2 * (size_t)sp * (Ls == 16 ? 12 * 17 : 12 * Ls + 1) * cs : 0;
+    const size_t epi = mat ? (size_t)sp * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
   if (lsc && lds_for(spb) <= lds_cap) {
