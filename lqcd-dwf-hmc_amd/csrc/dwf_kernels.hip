@@ -14,6 +14,8 @@
 // (spb sites x Ls threads), so the Ls-mixing epilogue runs through LDS inside
 // the block.  The Ls threads of a site read contiguous spinor memory and share
 // (by broadcast) the site's links.
+#include <type_traits>
+
 #include "dwf_device.h"
 #include "dwf_kernels.h"
 
@@ -71,11 +73,11 @@ __device__ __forceinline__ void project(const C* w, C* h1, C* h2) {
       h1[r] = w0[r] + w2[i]; h1[i] = w0[i] - w2[r];
       h2[r] = w1[r] - w3[i]; h2[i] = w1[i] + w3[r];
     } else if (DIR == 6) {  // h1 = 2 w2, h2 = 2 w3
-      h1[r] = C(2) * w2[r]; h1[i] = C(2) * w2[i];
-      h2[r] = C(2) * w3[r]; h2[i] = C(2) * w3[i];
+      h1[r] = w2[r] + w2[r]; h1[i] = w2[i] + w2[i];
+      h2[r] = w3[r] + w3[r]; h2[i] = w3[i] + w3[i];
     } else {                // h1 = 2 w0, h2 = 2 w1
-      h1[r] = C(2) * w0[r]; h1[i] = C(2) * w0[i];
-      h2[r] = C(2) * w1[r]; h2[i] = C(2) * w1[i];
+      h1[r] = w0[r] + w0[r]; h1[i] = w0[i] + w0[i];
+      h2[r] = w1[r] + w1[r]; h2[i] = w1[i] + w1[i];
     }
   }
 }
@@ -229,6 +231,107 @@ __device__ __forceinline__ void hop_compute(C (*acc)[24], const typename Vx<T, V
     reconstruct<DIR>(acc[v], a1, a2);
   }
 }
+// FP16 storage, two s per thread: the hop on packed halves.  The spin
+// projection runs on (s0, s1) pairs (v_pk_add_f16, one rounding to FP16 like
+// the stored field itself), the pairs are regrouped to (re, im) per s, and the
+// SU(3) product is a chain of v_dot2_f32_f16 with FP32 accumulation:
+//   Re(u h) += dot2((ur, -ui), (hr, hi)),  Im(u h) += dot2((ui, ur), (hr, hi)).
+// The block's links sit in LDS as those two words per entry, site-major with
+// kPkStride words per site (8 directions x 20 words, 18 used, + 4 pad words so
+// that the 16-byte reads of 8 sites of a wave cover all 32 banks once).
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+constexpr int kPkStride = 164;
+__device__ __forceinline__ f16x2_t as_f16x2(unsigned w) { return __builtin_bit_cast(f16x2_t, w); }
+__device__ __forceinline__ unsigned as_u32(f16x2_t h) { return __builtin_bit_cast(unsigned, h); }
+// first product of a chain: the three-operand form with an inline zero (the
+// builtin always becomes the accumulating two-operand form behind a register
+// move of the zero)
+__device__ __forceinline__ float dot2_first(f16x2_t a, f16x2_t b) {
+  float r;
+  asm("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+template <int DIR, bool DAG>
+__device__ __forceinline__ void hop_dir_pk(float (*acc)[24], const __half* __restrict__ in,
+                                           const unsigned* shP, const int* shN, int lsite,
+                                           int s0, int spb) {
+  constexpr int Ls = 16;
+  const int nb = shN[DIR * spb + lsite];
+  const __half* wp = in + (size_t)nb * 24 * Ls + s0;
+  f16x2_t w[24];
+  if (!DAG) {
+#pragma unroll
+    for (int k = 0; k < 24; ++k) w[k] = as_f16x2(*reinterpret_cast<const unsigned*>(wp + k * Ls));
+  } else {  // gamma_5 (Dirac basis) exchanges the upper and lower spin pairs
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      w[k] = as_f16x2(*reinterpret_cast<const unsigned*>(wp + (k + 12) * Ls));
+      w[k + 12] = as_f16x2(*reinterpret_cast<const unsigned*>(wp + k * Ls));
+    }
+  }
+  f16x2_t h1[6], h2[6];
+  project<DIR>(w, h1, h2);
+  // (re, im) of colour c for s0 (p[0]) and s0 + 1 (p[1])
+  unsigned p1[2][3], p2[2][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const unsigned r1 = as_u32(h1[2 * c]), i1 = as_u32(h1[2 * c + 1]);
+    const unsigned r2 = as_u32(h2[2 * c]), i2 = as_u32(h2[2 * c + 1]);
+    p1[0][c] = __builtin_amdgcn_perm(i1, r1, 0x05040100u);
+    p1[1][c] = __builtin_amdgcn_perm(i1, r1, 0x07060302u);
+    p2[0][c] = __builtin_amdgcn_perm(i2, r2, 0x05040100u);
+    p2[1][c] = __builtin_amdgcn_perm(i2, r2, 0x07060302u);
+  }
+  const u32x4_t* up = reinterpret_cast<const u32x4_t*>(shP + lsite * kPkStride + DIR * 20);
+  unsigned u[20];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const u32x4_t t = up[q];
+    u[4 * q] = t[0]; u[4 * q + 1] = t[1]; u[4 * q + 2] = t[2]; u[4 * q + 3] = t[3];
+  }
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    float a1[6], a2[6];
+#pragma unroll
+    for (int ia = 0; ia < 3; ++ia) {
+      float r1, i1, r2, i2;
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib) {
+        const f16x2_t X = as_f16x2(u[2 * (ib + 3 * ia)]), Y = as_f16x2(u[2 * (ib + 3 * ia) + 1]);
+        const f16x2_t q1 = as_f16x2(p1[v][ib]), q2 = as_f16x2(p2[v][ib]);
+        if (ib == 0) {
+          r1 = dot2_first(X, q1);
+          i1 = dot2_first(Y, q1);
+          r2 = dot2_first(X, q2);
+          i2 = dot2_first(Y, q2);
+        } else {
+          r1 = __builtin_amdgcn_fdot2(X, q1, r1, false);
+          i1 = __builtin_amdgcn_fdot2(Y, q1, i1, false);
+          r2 = __builtin_amdgcn_fdot2(X, q2, r2, false);
+          i2 = __builtin_amdgcn_fdot2(Y, q2, i2, false);
+        }
+      }
+      a1[2 * ia] = r1; a1[2 * ia + 1] = i1;
+      a2[2 * ia] = r2; a2[2 * ia + 1] = i2;
+    }
+    reconstruct<DIR>(acc[v], a1, a2);
+  }
+}
+template <bool DAG>
+__device__ __forceinline__ void hop_all_pk(float (*acc)[24], const __half* __restrict__ in,
+                                           const unsigned* shP, const int* shN, int lsite,
+                                           int s0, int spb) {
+  hop_dir_pk<0, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<1, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<2, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<3, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<4, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<5, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<6, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  hop_dir_pk<7, DAG>(acc, in, shP, shN, lsite, s0, spb);
+}
+
 template <bool DAG, typename T, int VEC, typename C>
 __device__ __forceinline__ void hop_all(C (*acc)[24], const T* __restrict__ in, const C* shL,
                                         const int* shN, int lsite, int s0, int spb, int Ls) {
@@ -323,6 +426,10 @@ template <> struct Q4<double> {
 #ifndef DWF_WAVES_F16
 #define DWF_WAVES_F16 3
 #endif
+// FP16 storage: the packed hop (dot2 SU(3) products) instead of the FP32 one
+#ifndef DWF_HOP_PK16
+#define DWF_HOP_PK16 1
+#endif
 #ifndef DWF_HOP_THREADS
 #define DWF_HOP_THREADS 256
 #endif
@@ -372,12 +479,15 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   // chiral rows of one site are 12*Ls + 1 apart, so the sites of a wave read
   // their rows from different LDS banks
   const int SU = 12 * Ls + 1;
-  const int stageC = (OP == K_SITE) ? 0 : (144 * spb + 8 * spb);
+  // packed FP16 hop (two s per thread, Ls = 16)
+  constexpr bool PK = std::is_same<T, __half>::value && VEC == 2 && LSC == 16 && DWF_HOP_PK16;
+  constexpr int LROW = PK ? kPkStride : 144;   // link words per site in LDS
+  const int stageC = (OP == K_SITE) ? 0 : (LROW * spb + 8 * spb);
   const int epiC = MAT ? (LSC == 16 ? spb * 12 * 17 : 2 * spb * SU) : 0;
   const int regionA = stageC > epiC ? stageC : epiC;
   C* shA = reinterpret_cast<C*>(smem_raw);
   C* shL = shA;
-  int* shN = reinterpret_cast<int*>(shA + 144 * spb);
+  int* shN = reinterpret_cast<int*>(shA + LROW * spb);
   C* sh_u = shA;
   C* sh_l = shA + spb * SU;
   C* sh_E = shA + regionA;
@@ -408,9 +518,27 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   if (OP != K_SITE) {
     // coalesced cooperative load of the block's 8 x 18 link rows and 8 neighbour rows
     const T* L = links + (size_t)parity * 8 * 18 * Vh + site0;
-    for (int i = tid; i < 144 * spb; i += blockDim.x) {
-      const int row = i / spb, j = i - row * spb;
-      shL[i] = (j < nsite) ? S::ldn(L + (size_t)row * Vh + j) : C(0);
+    if constexpr (PK) {
+      unsigned* shP = reinterpret_cast<unsigned*>(shA);
+      const unsigned short* Lh = reinterpret_cast<const unsigned short*>(L);
+      for (int i = tid; i < 72 * spb; i += blockDim.x) {
+        const int row = i / spb, j = i - row * spb;   // row = dir * 9 + entry
+        const int dir = row / 9, ab = row - dir * 9;
+        unsigned x = 0, y = 0;
+        if (j < nsite) {
+          const unsigned ur = __builtin_nontemporal_load(Lh + (size_t)(dir * 18 + 2 * ab) * Vh + j);
+          const unsigned ui = __builtin_nontemporal_load(Lh + (size_t)(dir * 18 + 2 * ab + 1) * Vh + j);
+          x = ur | ((ui ^ 0x8000u) << 16);   // (ur, -ui)
+          y = ui | (ur << 16);               // (ui, ur)
+        }
+        shP[j * kPkStride + dir * 20 + 2 * ab] = x;
+        shP[j * kPkStride + dir * 20 + 2 * ab + 1] = y;
+      }
+    } else {
+      for (int i = tid; i < 144 * spb; i += blockDim.x) {
+        const int row = i / spb, j = i - row * spb;
+        shL[i] = (j < nsite) ? S::ldn(L + (size_t)row * Vh + j) : C(0);
+      }
     }
     const int* N = nbr + (size_t)parity * 8 * Vh + site0;
     for (int i = tid; i < 8 * spb; i += blockDim.x) {
@@ -435,9 +563,17 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         for (int v = 0; v < VEC; ++v) acc[v][k] = VL::get(r, v);
       }
     } else if (OP == K_HOP) {
-      hop_all<false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      if constexpr (PK)
+        hop_all_pk<false>(acc, reinterpret_cast<const __half*>(in),
+                          reinterpret_cast<const unsigned*>(shA), shN, lsite, s0, spb);
+      else
+        hop_all<false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
     } else {
-      hop_all<true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+      if constexpr (PK)
+        hop_all_pk<true>(acc, reinterpret_cast<const __half*>(in),
+                         reinterpret_cast<const unsigned*>(shA), shN, lsite, s0, spb);
+      else
+        hop_all<true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
 #pragma unroll
       for (int v = 0; v < VEC; ++v)
 #pragma unroll
@@ -911,7 +1047,9 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
       mat ? 2 * (size_t)Ls * Ls * cs * (1 + (out2 ? 1 : 0) + (e.G ? 1 : 0)) : 0;
   const size_t lds_cap = 64 * 1024;
   auto lds_for = [&](int sp) {
-    const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * sp * cs;
+    const bool pk = std::is_same<T, __half>::value && KCfg<T>::VEC == 2 && Ls == 16 &&
+                    DWF_HOP_PK16;
+    const size_t stage = (op == K_SITE) ? 0 : (size_t)((pk ? kPkStride : 144) + 8) * sp * cs;
     const size_t epi = mat ? (size_t)sp * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
