@@ -59,6 +59,7 @@ struct Ctx {
   double* host_coef = nullptr;    // pinned
   int sh_cap = 0;
   hipEvent_t ring_ev[2] = {nullptr, nullptr};   // convergence checks of the sync-free CG
+  double* host_ring = nullptr;    // coherent pinned memory written by kernels (2 doubles)
 };
 static Ctx g_ctx;
 static std::mutex g_mtx;
@@ -78,6 +79,7 @@ static int ctx_init() {
   HIPCHK(hipMalloc(&g_ctx.red.result, sizeof(double) * 16));
   HIPCHK(hipHostMalloc(&g_ctx.host_result, sizeof(double) * 64, hipHostMallocDefault));
   for (auto& e : g_ctx.ring_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIPCHK(hipHostMalloc(&g_ctx.host_ring, sizeof(double) * 8, hipHostMallocCoherent));
   g_ctx.ok = true;
   return DWF_OK;
 }
@@ -223,6 +225,7 @@ enum MatId {
   M_M5F,        // -1/2 Mp: M5 weight of the force (aforce_F_Domainwall_5din-tmpl.h:355-365)
   M_HE2,        // -M_E2: second hop of H = 1 - D
   M_HG2,        // -M_G2: second hop of H^+ = 1 - D^+
+  M_ID,         // 1 (site launches that only add a multiple of a field)
   M_COUNT
 };
 
@@ -366,6 +369,9 @@ static int build_mats(dwf_fopr_s* op) {
   ms[M_M5F] = cm_scale(Mp, -0.5L);
   ms[M_HE2] = cm_scale(ms[M_E2], -1.0L);
   ms[M_HG2] = cm_scale(ms[M_G2], -1.0L);
+  ChiMat Id(L);
+  for (int s = 0; s < L; ++s) Id.at(0, s, s) = Id.at(1, s, s) = 1.0L;
+  ms[M_ID] = Id;
   return upload_mats(op, ms);
 }
 
@@ -473,7 +479,9 @@ static int op_DdagD(dwf_fopr_s* op, void* v, const void* w) {
 }
 // DdagD with |D w|^2 (= <w, DdagD w>) reduced into the device scalar *nrm: the
 // norm is fused into the epilogue of the last hop of D
-static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm) {
+// mpw: Mp w already computed by the caller (non-spin-scalar Mp), or nullptr.
+static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm,
+                        const void* mpw = nullptr) {
   void *t0, *t1, *u, *u2;
   RC(get_work(op, 0, &t0));
   RC(get_work(op, 1, &t1));
@@ -481,7 +489,9 @@ static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm) {
   RC(get_work(op, 3, &u2));
   if (!op->hop_partial) HIPCHK(hipMalloc(&op->hop_partial, (size_t)op->Vh * sizeof(double)));
   const void* src = w;
-  if (!op->mp_scalar) {
+  if (mpw) {
+    src = mpw;
+  } else if (!op->mp_scalar) {
     RC(run_site(op, t0, w, M_PRE));
     src = t0;
   }
@@ -497,6 +507,44 @@ static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm) {
   HIPCHK(launch_dwf(op->prec, K_HOP, u, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
                     g_ctx.stream));
   return op_Ddag(op, v, u, u2);
+}
+// One CG step's operator part with the residual update fused into the last
+// hop (Ls = 16): mpw = Mp p (non-spin-scalar Mp) or p itself (Mp folded into
+// the hops); |D p|^2 -> *pap; then r -= (*rr / *pap) DdagD p, and |r|^2 of the
+// new r -> *rr_new (and *rr_host, coherent host memory).  DdagD p itself is
+// not stored.
+static int op_DdagD_cg(dwf_fopr_s* op, const void* p, const void* mpw, double* pap, void* r,
+                       const double* rr, double* rr_new, double* rr_host) {
+  void *t1, *u, *u2;
+  RC(get_work(op, 1, &t1));
+  RC(get_work(op, 2, &u));
+  RC(get_work(op, 3, &u2));
+  if (!op->hop_partial) HIPCHK(hipMalloc(&op->hop_partial, (size_t)op->Vh * sizeof(double)));
+  RC(run_hop(op, false, 1, t1, mpw, M_E1));
+  Epi e;
+  e.E = mat(op, M_E2);
+  e.X = p;
+  e.xcoef = 1.0;
+  e.out2 = u2;
+  e.F = mat(op, M_DIT);
+  e.nrm_partial = op->hop_partial;
+  e.nrm_result = pap;
+  HIPCHK(launch_dwf(op->prec, K_HOP, u, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
+                    g_ctx.stream));
+  RC(run_hop(op, true, 1, t1, u2, M_G1));
+  Epi g;
+  g.E = mat(op, M_G2);
+  g.X = u;
+  g.xcoef = 1.0;
+  g.upd = r;
+  g.unum = rr;
+  g.uden = pap;
+  g.nrm_partial = op->hop_partial;
+  g.nrm_result = rr_new;
+  g.nrm_host = rr_host;
+  HIPCHK(launch_dwf(op->prec, K_HOPDAG, nullptr, t1, op->links, op->nbr, 0, op->Vh, op->Ls, g,
+                    g_ctx.stream));
+  return DWF_OK;
 }
 static int op_DDdag(dwf_fopr_s* op, void* v, const void* w) {
   void* u;
@@ -1119,8 +1167,16 @@ static int cg_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out, do
 static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
                          double* diff_out) {
   dwf_fopr op = s->op;
-  for (int i = 0; i < 4; ++i) RC(solver_field(op, &s->f[i]));
-  void *x = s->f[0], *r = s->f[1], *p = s->f[2], *q = s->f[3];
+  // Non-spin-scalar Mp: the update p = r + beta p runs as a site launch that
+  // also writes Mp p, the first factor of the next D p, into q (free between
+  // the x/r update and the next operator); p alternates between two fields.
+  // At Ls = 16 the r update moves into the last hop (fused matrix-core
+  // epilogue, DdagD p is never stored) and the x update into that site launch,
+  // or, with Mp folded into the hops, into one BLAS pass with the p update.
+  const bool pre = !op->mp_scalar;
+  const bool fused = op->Ls == 16;
+  for (int i = 0; i < (pre ? 5 : 4); ++i) RC(solver_field(op, &s->f[i]));
+  void *x = s->f[0], *r = s->f[1], *p = s->f[2], *q = s->f[3], *p2 = s->f[4];
   const int pr = op->prec;
   const size_t n = op->nelem;
   hipStream_t st = g_ctx.stream;
@@ -1141,17 +1197,76 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
   HIPCHK(launch_copy(pr, p, b, n, st));
   HIPCHK(launch_set_zero(pr, x, n, st));
   HIPCHK(launch_norm2(pr, r, n, g_ctx.red, RA, st));
-  double* ring = g_ctx.host_result + 32;
+  if (pre) RC(run_site(op, q, p, M_PRE));
+  // |r|^2 of iteration i reaches the host in ring[i & 1]: written by the
+  // finishing reduction kernel itself (fused path) or copied
+  double* ring = fused ? g_ctx.host_ring : g_ctx.host_result + 32;
+  // The fused path needs no event: the host polls its slot, set to -1 before
+  // the iteration is queued (|r|^2 >= 0), and watches the stream for errors.
+  auto wait_ring = [&](int slot) -> int {
+    if (!fused) {
+      HIPCHK(hipEventSynchronize(g_ctx.ring_ev[slot]));
+      return DWF_OK;
+    }
+    volatile double* v = ring + slot;
+    for (unsigned spin = 1;; ++spin) {
+      if (!(*v < 0)) return DWF_OK;
+      if ((spin & 1023) == 0) {
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipSuccess) {
+          if (!(*v < 0)) return DWF_OK;
+          return fail(DWF_ERR_HIP, "CG: residual norm not delivered");
+        }
+        if (q != hipErrorNotReady) return fail(DWF_ERR_HIP, hipGetErrorString(q));
+      }
+    }
+  };
   int cur = RA, nxt = RB, nconv = -1, iter = 0;
   for (; iter < s->niter; ++iter) {
-    RC(op_DdagD_nrm(op, q, p, res + PAP));
-    HIPCHK(launch_cg_xr_dev(pr, x, r, p, q, res + cur, res + PAP, n, g_ctx.red, nxt, st));
-    HIPCHK(launch_aypx_dev(pr, p, r, res + nxt, res + cur, n, st));
-    HIPCHK(hipMemcpyAsync(ring + (iter & 1), res + nxt, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipEventRecord(g_ctx.ring_ev[iter & 1], st));
+    if (fused) ring[iter & 1] = -1.0;
+    if (fused && !pre) {
+      RC(op_DdagD_cg(op, p, p, res + PAP, r, res + cur, res + nxt, ring + (iter & 1)));
+      HIPCHK(launch_cg_px_dev(pr, x, p, r, res + cur, res + PAP, res + nxt, n, st));
+    } else if (fused) {
+      RC(op_DdagD_cg(op, p, q, res + PAP, r, res + cur, res + nxt, ring + (iter & 1)));
+      Epi e;   // x += (|r|^2 / |D p|^2) p,  p2 = r + (|r_new|^2 / |r|^2) p,  q = Mp p2
+      e.E = mat(op, M_ID);
+      e.X = p;
+      e.xnum = res + nxt;
+      e.xden = res + cur;
+      e.out2 = q;
+      e.F = mat(op, M_PRE);
+      e.upd = x;
+      e.unum = res + cur;
+      e.uden = res + PAP;
+      HIPCHK(launch_dwf(op->prec, K_SITE, p2, r, op->links, op->nbr, 0, op->Vh, op->Ls, e, st));
+      std::swap(p, p2);
+    } else {
+      RC(op_DdagD_nrm(op, q, p, res + PAP, pre ? q : nullptr));
+      HIPCHK(launch_cg_xr_dev(pr, x, r, p, q, res + cur, res + PAP, n, g_ctx.red, nxt, st));
+      if (pre) {
+        Epi e;   // p2 = r + (|r_new|^2 / |r|^2) p,  q = Mp p2
+        e.E = mat(op, M_ID);
+        e.X = p;
+        e.xnum = res + nxt;
+        e.xden = res + cur;
+        e.out2 = q;
+        e.F = mat(op, M_PRE);
+        HIPCHK(launch_dwf(op->prec, K_SITE, p2, r, op->links, op->nbr, 0, op->Vh, op->Ls, e,
+                          st));
+        std::swap(p, p2);
+      } else {
+        HIPCHK(launch_aypx_dev(pr, p, r, res + nxt, res + cur, n, st));
+      }
+    }
+    if (!fused) {
+      HIPCHK(hipMemcpyAsync(ring + (iter & 1), res + nxt, sizeof(double), hipMemcpyDeviceToHost,
+                            st));
+      HIPCHK(hipEventRecord(g_ctx.ring_ev[iter & 1], st));
+    }
     std::swap(cur, nxt);
     if (iter >= 1) {
-      HIPCHK(hipEventSynchronize(g_ctx.ring_ev[(iter - 1) & 1]));
+      RC(wait_ring((iter - 1) & 1));
       if (ring[(iter - 1) & 1] * snorm < s->stop) {
         nconv = iter;   // the test passed after iteration iter - 1 (0-based)
         ++iter;         // iterations run: iter + 1
@@ -1160,7 +1275,7 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
     }
   }
   if (nconv < 0 && iter == s->niter && iter >= 1) {   // the last iteration's test
-    HIPCHK(hipEventSynchronize(g_ctx.ring_ev[(iter - 1) & 1]));
+    RC(wait_ring((iter - 1) & 1));
     if (ring[(iter - 1) & 1] * snorm < s->stop) nconv = iter;
   }
   HIPCHK(launch_copy(pr, xq, x, n, st));

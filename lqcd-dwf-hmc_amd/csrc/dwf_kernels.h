@@ -38,6 +38,16 @@ struct Epi {
   const void* X2 = nullptr;   // with out2: out2 = F out + X2 (X2 may alias out2)
   double* nrm_partial = nullptr;   // with E: |out|^2 -> nrm_result (block partials, >= Vh doubles)
   double* nrm_result = nullptr;
+  double* nrm_host = nullptr;      // with nrm_*: also written to this coherent host memory
+  // with X: xcoef = xnum[0] / xden[0], read by the kernel from device memory
+  const double* xnum = nullptr;
+  const double* xden = nullptr;
+  // CG updates fused into the epilogue (Ls = 16 only), a = unum[0] / uden[0]:
+  // K_SITE with X: upd += a X (besides out, out2);  hop without out2:
+  // upd -= a out, written instead of out, with nrm_* taken of the new upd
+  void* upd = nullptr;
+  const double* unum = nullptr;
+  const double* uden = nullptr;
 };
 
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
@@ -103,6 +113,9 @@ hipError_t launch_cg_xr_dev(int prec, void* x, void* r, const void* p, const voi
                             const RedScratch& red, int slot, hipStream_t st);
 hipError_t launch_aypx_dev(int prec, void* y, const void* x, const double* num, const double* den,
                            size_t n, hipStream_t st);
+// x += (rr / pap) p,  p = r + (rr_new / rr) p  (scalars in device memory)
+hipError_t launch_cg_px_dev(int prec, void* x, void* p, const void* r, const double* rr,
+                            const double* pap, const double* rr_new, size_t n, hipStream_t st);
 
 // Multishift update of active shifts j = 1..m-1 (arrays of device pointers on
 // the device; coefficients on the device, 3 per shift: -beta_s, alpha_s, zeta):

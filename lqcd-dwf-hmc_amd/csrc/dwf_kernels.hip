@@ -253,13 +253,11 @@ __device__ __forceinline__ float dot2_first(f16x2_t a, f16x2_t b) {
 }
 
 template <int DIR, bool DAG>
-__device__ __forceinline__ void hop_dir_pk(float (*acc)[24], const __half* __restrict__ in,
-                                           const unsigned* shP, const int* shN, int lsite,
-                                           int s0, int spb) {
+__device__ __forceinline__ void pk_load(f16x2_t (&w)[24], const __half* __restrict__ in,
+                                        const int* shN, int lsite, int s0, int spb) {
   constexpr int Ls = 16;
   const int nb = shN[DIR * spb + lsite];
   const __half* wp = in + (size_t)nb * 24 * Ls + s0;
-  f16x2_t w[24];
   if (!DAG) {
 #pragma unroll
     for (int k = 0; k < 24; ++k) w[k] = as_f16x2(*reinterpret_cast<const unsigned*>(wp + k * Ls));
@@ -270,6 +268,10 @@ __device__ __forceinline__ void hop_dir_pk(float (*acc)[24], const __half* __res
       w[k + 12] = as_f16x2(*reinterpret_cast<const unsigned*>(wp + k * Ls));
     }
   }
+}
+template <int DIR>
+__device__ __forceinline__ void pk_compute(float (*acc)[24], const f16x2_t (&w)[24],
+                                           const unsigned* shP, int lsite) {
   f16x2_t h1[6], h2[6];
   project<DIR>(w, h1, h2);
   // (re, im) of colour c for s0 (p[0]) and s0 + 1 (p[1])
@@ -318,18 +320,43 @@ __device__ __forceinline__ void hop_dir_pk(float (*acc)[24], const __half* __res
     reconstruct<DIR>(acc[v], a1, a2);
   }
 }
+#ifndef DWF_HOP_PIPE16
+#define DWF_HOP_PIPE16 0
+#endif
 template <bool DAG>
 __device__ __forceinline__ void hop_all_pk(float (*acc)[24], const __half* __restrict__ in,
                                            const unsigned* shP, const int* shN, int lsite,
                                            int s0, int spb) {
-  hop_dir_pk<0, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<1, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<2, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<3, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<4, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<5, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<6, DAG>(acc, in, shP, shN, lsite, s0, spb);
-  hop_dir_pk<7, DAG>(acc, in, shP, shN, lsite, s0, spb);
+  f16x2_t wa[24];
+#if DWF_HOP_PIPE16
+  // the next direction's loads are in flight while the current one is computed
+  f16x2_t wb[24];
+  pk_load<0, DAG>(wa, in, shN, lsite, s0, spb);
+  pk_load<1, DAG>(wb, in, shN, lsite, s0, spb);
+  pk_compute<0>(acc, wa, shP, lsite);
+  pk_load<2, DAG>(wa, in, shN, lsite, s0, spb);
+  pk_compute<1>(acc, wb, shP, lsite);
+  pk_load<3, DAG>(wb, in, shN, lsite, s0, spb);
+  pk_compute<2>(acc, wa, shP, lsite);
+  pk_load<4, DAG>(wa, in, shN, lsite, s0, spb);
+  pk_compute<3>(acc, wb, shP, lsite);
+  pk_load<5, DAG>(wb, in, shN, lsite, s0, spb);
+  pk_compute<4>(acc, wa, shP, lsite);
+  pk_load<6, DAG>(wa, in, shN, lsite, s0, spb);
+  pk_compute<5>(acc, wb, shP, lsite);
+  pk_load<7, DAG>(wb, in, shN, lsite, s0, spb);
+  pk_compute<6>(acc, wa, shP, lsite);
+  pk_compute<7>(acc, wb, shP, lsite);
+#else
+  pk_load<0, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<0>(acc, wa, shP, lsite);
+  pk_load<1, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<1>(acc, wa, shP, lsite);
+  pk_load<2, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<2>(acc, wa, shP, lsite);
+  pk_load<3, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<3>(acc, wa, shP, lsite);
+  pk_load<4, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<4>(acc, wa, shP, lsite);
+  pk_load<5, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<5>(acc, wa, shP, lsite);
+  pk_load<6, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<6>(acc, wa, shP, lsite);
+  pk_load<7, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<7>(acc, wa, shP, lsite);
+#endif
 }
 
 template <bool DAG, typename T, int VEC, typename C>
@@ -415,6 +442,14 @@ template <> struct Q4<double> {
   }
 };
 
+// value of v after a round trip through the storage type
+template <typename T>
+__device__ __forceinline__ typename St<T>::C rounded(typename St<T>::C v) {
+  T t;
+  St<T>::st(&t, v);
+  return St<T>::ld(&t);
+}
+
 // Occupancy target (waves per SIMD) of the fused kernel per storage type; the
 // compiler caps registers to reach it.
 #ifndef DWF_WAVES_F64
@@ -441,14 +476,20 @@ template <> struct Q4<double> {
 #endif
 // Per storage type: s-values per thread (VEC, one vector access per spinor
 // component), threads per block and the occupancy target.
+#ifndef DWF_HOP_THREADS16
+#define DWF_HOP_THREADS16 256
+#endif
 template <typename T> struct KCfg {   // 16-bit storage
   static constexpr int VEC = DWF_VEC_F16;
-  static constexpr int THREADS = 128;
+  static constexpr int THREADS = DWF_HOP_THREADS16;
   static constexpr int WAVES = DWF_WAVES_F16;
 };
+#ifndef DWF_HOP_THREADS32
+#define DWF_HOP_THREADS32 DWF_HOP_THREADS
+#endif
 template <> struct KCfg<float> {
   static constexpr int VEC = DWF_VEC_F32;
-  static constexpr int THREADS = DWF_HOP_THREADS;
+  static constexpr int THREADS = DWF_HOP_THREADS32;
   static constexpr int WAVES = DWF_WAVES_F32;
 };
 template <> struct KCfg<double> {
@@ -467,12 +508,20 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
       const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
       const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
-      const T* X2, double* __restrict__ nrm) {
+      const T* X2, double* __restrict__ nrm, const double* xnum, const double* xden,
+      T* __restrict__ upd, const double* unum, const double* uden) {
   using S = St<T>;
   using C = typename S::C;
   using VL = Vx<T, VEC>;
   const int Ls = LSC > 0 ? LSC : Ls_rt;
+  // coefficient of X: a kernel argument, or num / den read from device memory
+  const C xc = xnum ? (C)(xnum[0] / xden[0]) : xcoef;
+  // CG updates fused into the matrix-core epilogue (Ls = 16), ua = unum / uden:
+  // site launch: upd += ua X;  hop launch: upd -= ua out, stored instead of out,
+  // and the norm taken of the new upd
+  const C ua = upd ? (C)(unum[0] / uden[0]) : C(0);
   const int Q = Ls / VEC;   // threads per site
+  static_assert(KCfg<T>::THREADS % 64 == 0 && KCfg<T>::THREADS <= 512, "block of 1-8 waves");
   const int spb = LSC > 0 ? KCfg<T>::THREADS / (LSC / VEC) : spb_rt;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // region A: hop staging (links, neighbours), later the chiral epilogue buffers
@@ -595,7 +644,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         if (ADDX) {
           const typename VL::R r = VL::ld(X + base + k * Ls);
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) o[v] += xcoef * VL::get(r, v);
+          for (int v = 0; v < VEC; ++v) o[v] += xc * VL::get(r, v);
         }
         VL::stn(out + base + k * Ls, o);
       }
@@ -613,7 +662,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     using v4 = typename MM::v4;
     constexpr int LP = 17;
     constexpr int NW = KCfg<T>::THREADS / 64;      // waves per block
-    constexpr int NTW = (12 + NW - 1) / NW;        // tiles per wave (12 tiles)
+    constexpr int NT = KCfg<T>::THREADS / (16 / VEC) * 12 / 16;   // 16-column tiles per block
+    constexpr int NTW = (NT + NW - 1) / NW;        // tiles per wave
     C* mb = shA;
     const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -701,15 +751,41 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             Q4<T>::ld(X + ob + 12 * Ls, xb);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              o1[r] += xcoef * xa[r];
-              o2[r] += xcoef * xb[r];
+              o1[r] += xc * xa[r];
+              o2[r] += xc * xb[r];
+            }
+            if (OP == K_SITE && upd) {
+              C ya[4], yb[4];
+              Q4<T>::ld(upd + ob, ya);
+              Q4<T>::ld(upd + ob + 12 * Ls, yb);
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                ya[r] += ua * xa[r];
+                yb[r] += ua * xb[r];
+              }
+              Q4<T>::stn(upd + ob, ya);
+              Q4<T>::stn(upd + ob + 12 * Ls, yb);
             }
           }
-          Q4<T>::stn(out + ob, o1);
-          Q4<T>::stn(out + ob + 12 * Ls, o2);
+          if (OP != K_SITE && upd) {
+            C ya[4], yb[4];
+            Q4<T>::ld(upd + ob, ya);
+            Q4<T>::ld(upd + ob + 12 * Ls, yb);
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            nacc += (double)o1[r] * (double)o1[r] + (double)o2[r] * (double)o2[r];
+            for (int r = 0; r < 4; ++r) {
+              ya[r] = rounded<T>(ya[r] - ua * o1[r]);
+              yb[r] = rounded<T>(yb[r] - ua * o2[r]);
+              nacc += (double)ya[r] * (double)ya[r] + (double)yb[r] * (double)yb[r];
+            }
+            Q4<T>::stn(upd + ob, ya);
+            Q4<T>::stn(upd + ob + 12 * Ls, yb);
+          } else {
+            Q4<T>::stn(out + ob, o1);
+            Q4<T>::stn(out + ob + 12 * Ls, o2);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              nacc += (double)o1[r] * (double)o1[r] + (double)o2[r] * (double)o2[r];
+          }
         }
         if (OUT2) {
 #pragma unroll
@@ -880,7 +956,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       for (int k = 0; k < 24; ++k) {
         const typename VL::R r = VL::ld(X + base + k * Ls);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) o[v][k] += xcoef * VL::get(r, v);
+        for (int v = 0; v < VEC; ++v) o[v][k] += xc * VL::get(r, v);
       }
     }
 #pragma unroll
@@ -987,7 +1063,8 @@ static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* ou
   hipLaunchKernelGGL((k_dwf<T, OP, MAT, ADDX, OUT2, LSC, VEC>), grid, block, lds, st, (T*)out,
                      (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
                      (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F,
-                     (const C*)e.G, (const T*)e.X2, e.nrm_partial);
+                     (const C*)e.G, (const T*)e.X2, e.nrm_partial, e.xnum, e.xden, (T*)e.upd,
+                     e.unum, e.uden);
 }
 
 // the (op, epilogue) combinations the operator chains use
@@ -999,9 +1076,15 @@ static hipError_t launch_combo(int op, bool mat, bool addx, bool out2, dim3 g, d
 #define LK(OPV, M, A, O2) \
   launch_k<T, OPV, M, A, O2, LSC>(g, b, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e)
   if (op == K_SITE) {
-    if (!mat || addx) return hipErrorInvalidValue;
-    if (out2) LK(K_SITE, true, false, true);
-    else LK(K_SITE, true, false, false);
+    if (!mat) return hipErrorInvalidValue;
+    if (addx) {
+      if (!out2) return hipErrorInvalidValue;
+      LK(K_SITE, true, true, true);
+    } else if (out2) {
+      LK(K_SITE, true, false, true);
+    } else {
+      LK(K_SITE, true, false, false);
+    }
   } else if (op == K_HOP) {
     if (!mat) {
       if (addx || out2) return hipErrorInvalidValue;
@@ -1025,7 +1108,8 @@ static hipError_t launch_combo(int op, bool mat, bool addx, bool out2, dim3 g, d
   return hipGetLastError();
 }
 
-static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st);
+static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st,
+                        double* host);
 
 template <typename T>
 static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* links,
@@ -1053,6 +1137,10 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
     const size_t epi = mat ? (size_t)sp * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
+  // the fused CG updates exist in the matrix-core epilogue only
+  if (e.upd && !(mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap &&
+                 (op == K_SITE ? addx : !out2)))
+    return hipErrorInvalidValue;
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
     const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
@@ -1089,7 +1177,7 @@ hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* l
       break;
   }
   if (err == hipSuccess && e.nrm_partial) {
-    finish_norm(e.nrm_partial, nb, e.nrm_result, st);
+    finish_norm(e.nrm_partial, nb, e.nrm_result, st, e.nrm_host);
     err = hipGetLastError();
   }
   return err;
@@ -1505,7 +1593,7 @@ __device__ void block_finish(double (&v)[NR], double* partial, unsigned* counter
 template <int NR>
 __global__ void __launch_bounds__(256)
 k_red_finish(const double* __restrict__ partial, unsigned nblocks, size_t pstride,
-             double* __restrict__ result, int rstride) {
+             double* __restrict__ result, int rstride, double* host) {
   const double* p = partial + (size_t)blockIdx.y * pstride;
   double w[NR];
 #pragma unroll
@@ -1523,8 +1611,12 @@ k_red_finish(const double* __restrict__ partial, unsigned nblocks, size_t pstrid
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
-      result[(size_t)blockIdx.y * rstride + r] = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
+    for (int r = 0; r < NR; ++r) {
+      const double t = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
+      result[(size_t)blockIdx.y * rstride + r] = t;
+      if (host) host[(size_t)blockIdx.y * rstride + r] = t;   // coherent host memory
+    }
+    if (host) __threadfence_system();
   }
 }
 
@@ -1533,12 +1625,13 @@ static void red_finish(const double* partial, unsigned nblocks, double* result, 
                        unsigned ny = 1, size_t pstride = 0, int rstride = 0) {
   if (!kRedTwoKernels) return;
   hipLaunchKernelGGL(k_red_finish<NR>, dim3(1, ny), dim3(256), 0, st, partial, nblocks, pstride,
-                     result, rstride);
+                     result, rstride, (double*)nullptr);
 }
 
-static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st) {
+static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st,
+                        double* host) {
   hipLaunchKernelGGL(k_red_finish<1>, dim3(1, 1), dim3(256), 0, st, partial, nblocks, (size_t)0,
-                     result, 0);
+                     result, 0, host);
 }
 
 static dim3 red_grid(size_t n) {
@@ -1548,13 +1641,6 @@ static dim3 red_grid(size_t n) {
   return dim3((unsigned)g);
 }
 
-// value of v after a round trip through the storage type
-template <typename T>
-__device__ __forceinline__ typename St<T>::C rounded(typename St<T>::C v) {
-  T t;
-  St<T>::st(&t, v);
-  return St<T>::ld(&t);
-}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -1827,6 +1913,46 @@ hipError_t launch_cg_xr_dev(int prec, void* x, void* r, const void* p, const voi
 hipError_t launch_aypx_dev(int prec, void* y, const void* x, const double* num, const double* den,
                            size_t n, hipStream_t st) {
   DWF_DISPATCH(prec, aypx_dev_t, (y, x, num, den, n, st));
+  return hipGetLastError();
+}
+
+// The rest of a CG step once r (and |r|^2) are updated:
+//   x += (rr[0] / pap[0]) p,  p = r + (rr_new[0] / rr[0]) p
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_cg_px_dev(T* __restrict__ x, T* __restrict__ p, const T* __restrict__ r, const double* rr,
+            const double* pap, const double* rr_new, size_t n) {
+  using C = typename St<T>::C;
+  const C a = (C)(rr[0] / pap[0]);
+  const C b = (C)(rr_new[0] / rr[0]);
+  PACK_LOOP(T, n) {
+    C xv[NV], pv[NV], rv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(p + ip * NV, pv);
+    Vec<T>::ld(r + ip * NV, rv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      xv[j] += a * pv[j];
+      pv[j] = rv[j] + b * pv[j];
+    }
+    Vec<T>::st(x + ip * NV, xv);
+    Vec<T>::st(p + ip * NV, pv);
+  }
+  TAIL_LOOP(n) {
+    const C pv = St<T>::ld(p + i);
+    St<T>::st(x + i, St<T>::ld(x + i) + a * pv);
+    St<T>::st(p + i, St<T>::ld(r + i) + b * pv);
+  }
+}
+template <typename T>
+static void cg_px_dev_t(void* x, void* p, const void* r, const double* rr, const double* pap,
+                        const double* rr_new, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_px_dev<T>, grid_for(n), dim3(256), 0, st, (T*)x, (T*)p, (const T*)r,
+                     rr, pap, rr_new, n);
+}
+hipError_t launch_cg_px_dev(int prec, void* x, void* p, const void* r, const double* rr,
+                            const double* pap, const double* rr_new, size_t n, hipStream_t st) {
+  DWF_DISPATCH(prec, cg_px_dev_t, (x, p, r, rr, pap, rr_new, n, st));
   return hipGetLastError();
 }
 hipError_t launch_axpy_dot(int prec, void* y, double a, const void* x, size_t n,

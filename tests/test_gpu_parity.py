@@ -169,6 +169,29 @@ def test_richardson_mixed_precision_matches_oracle():
     assert abs(n - no) <= 1
 
 
+@pytest.mark.parametrize("inner", ["half", "float", "bfloat16"])
+@pytest.mark.parametrize("prm", LS16, ids=["shamir", "mobius"])
+def test_richardson_low_precision_inner_ls16(prm, inner):
+    """Mixed solve with the sync-free inner CG at Ls = 16 (matrix-core
+    epilogue, fused p update + Mp prologue): the FP64 residual reaches the
+    stopping condition and the solution is the FP64 CG one."""
+    orc, gpu, U = pair(prm)
+    gf = dwf_amd.AFopr_Domainwall_5din_eo(prm[0], prm[1], prm[2], prm[3], prm[4], prm[5],
+                                          prm[6], inner)
+    gf.set_config(U)
+    b = lattice.random_spinor5(prm[0], prm[1], 18)
+    gpu.set_mode("DdagD")
+    fb, fx, fy = gpu.new_field(), gpu.new_field(), gpu.new_field()
+    gpu.convert(fb, b)
+    n, d = dwf_amd.ASolver_Richardson(gpu, gf, 100, 1e-24, 10000, 1e-4).solve(fx, fb)
+    assert n >= 2 and d < 1e-12
+    x = gpu.reverse(fx)
+    true = np.linalg.norm(orc.mult("DdagD", x) - b) / np.linalg.norm(b)
+    assert true < 1e-11
+    n2, d2 = dwf_amd.ASolver_CG(gpu, 10000, 1e-24).solve(fy, fb)
+    assert rel(x, gpu.reverse(fy)) < 1e-7
+
+
 def test_multishift_matches_oracle():
     prm = ((4, 4, 4, 4), 6, 0.05, 1.6, 1.0, 0.0, (1, 1, 1, -1))
     orc, gpu, U = pair(prm)
