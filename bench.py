@@ -236,6 +236,17 @@ def main():
         cg = dwf_amd.ASolver_CG(f64, 100000, TOL2, "ZERO")
         t_fp64, fp64_iters, fp64_diff, _ = timed(cg)
         log(f"plain FP64 CG: {t_fp64:.3f} s, {fp64_iters} iterations, |r|/|b|={fp64_diff:.3e}")
+    # the same sync-free fused CG iteration in FP64 (one outer step of the FP64
+    # Richardson with an FP64 inner operator solving to the full tolerance): the
+    # FP64 baseline without host round trips
+    t_fp64f = n_fp64f = i_fp64f = None
+    if not a.no_fp64_cg:
+        f64b = dwf_amd.AFopr_Domainwall_5din_eo(DIMS, LS, MQ, M0, B, CC, (1, 1, 1, -1), "double")
+        f64b.set_config_smeared(sm, None)
+        f64b.set_mode("DdagD")
+        rich64 = dwf_amd.ASolver_Richardson(f64, f64b, 100, TOL2, 100000, TOL2)
+        t_fp64f, n_fp64f, _, i_fp64f = timed(rich64)
+        log(f"FP64 sync-free fused CG: {t_fp64f:.3f} s, outer={n_fp64f}, inner={i_fp64f}")
     ref = dwf_amd.ASolver_Richardson(f64, f32, 100, TOL2, 100000, INNER_TOL2)
     t_ref, n_ref, d_ref, i_ref = timed(ref)
     log(f"reference scheme FP64/FP32 (inner 1e-8): {t_ref:.3f} s, outer={n_ref}, inner={i_ref}")
@@ -281,6 +292,8 @@ def main():
         "time_to_solution_s": {
             "fp64_cg": None if t_fp64 is None else round(t_fp64, 4),
             "fp64_cg_iterations": fp64_iters,
+            "fp64_cg_sync_free_fused": None if t_fp64f is None else round(t_fp64f, 4),
+            "fp64_cg_sync_free_fused_iterations": i_fp64f,
             "mixed_fp64_fp16_storage": round(t_mixed, 4),
             "mixed_fp64_fp16_outer_inner": [results[-1][0], results[-1][2]],
             "reference_scheme_fp64_fp32_inner_1e-8": round(t_ref, 4),
@@ -288,6 +301,8 @@ def main():
             "learned_training": round(t_train, 4),
             "learned_outer_inner": [n_lpc, i_lpc],
             "speedup_mixed_vs_fp64": None if t_fp64 is None else round(t_fp64 / t_mixed, 3),
+            "speedup_mixed_vs_fp64_sync_free_fused":
+                None if t_fp64f is None else round(t_fp64f / t_mixed, 3),
             "speedup_reference_scheme_vs_fp64": None if t_fp64 is None else round(t_fp64 / t_ref, 3),
             "speedup_learned_vs_fp64_incl_training":
                 None if t_fp64 is None else round(t_fp64 / (t_lpc + t_train), 3),

@@ -515,12 +515,13 @@ static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm,
 // not stored.
 static int op_DdagD_cg(dwf_fopr_s* op, const void* p, const void* mpw, double* pap, void* r,
                        const double* rr, double* rr_new, double* rr_host) {
-  void *t1, *u, *u2;
+  void *t1, *u2;
   RC(get_work(op, 1, &t1));
-  RC(get_work(op, 2, &u));
   RC(get_work(op, 3, &u2));
   if (!op->hop_partial) HIPCHK(hipMalloc(&op->hop_partial, (size_t)op->Vh * sizeof(double)));
   RC(run_hop(op, false, 1, t1, mpw, M_E1));
+  // u = D p is needed only through u2 = Di^T u (the prologue of D^dag) and as
+  // the added term of the last hop, which takes it as Dee^T u2: u is never stored
   Epi e;
   e.E = mat(op, M_E2);
   e.X = p;
@@ -529,13 +530,13 @@ static int op_DdagD_cg(dwf_fopr_s* op, const void* p, const void* mpw, double* p
   e.F = mat(op, M_DIT);
   e.nrm_partial = op->hop_partial;
   e.nrm_result = pap;
-  HIPCHK(launch_dwf(op->prec, K_HOP, u, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
+  HIPCHK(launch_dwf(op->prec, K_HOP, nullptr, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
                     g_ctx.stream));
   RC(run_hop(op, true, 1, t1, u2, M_G1));
   Epi g;
   g.E = mat(op, M_G2);
-  g.X = u;
-  g.xcoef = 1.0;
+  g.X = u2;
+  g.G = mat(op, M_DEET);
   g.upd = r;
   g.unum = rr;
   g.uden = pap;
@@ -1188,7 +1189,7 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
   if (!(sr > 0)) {
     HIPCHK(launch_set_zero(pr, xq, n, st));
     *nconv_out = 0;
-    *diff_out = 0;
+    if (diff_out) *diff_out = 0;
     s->inner_iters = 0;
     return DWF_OK;
   }
@@ -1279,13 +1280,15 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
     if (ring[(iter - 1) & 1] * snorm < s->stop) nconv = iter;
   }
   HIPCHK(launch_copy(pr, xq, x, n, st));
-  RC(mult_impl(op, q, xq, op->mode, false));
-  double diff2;
-  HIPCHK(launch_sub_norm2(pr, q, q, b, n, g_ctx.red, 0, st));
-  RC(fetch(1, &diff2));
   *nconv_out = nconv;
-  *diff_out = std::sqrt(diff2 * snorm);
   s->inner_iters = iter;
+  if (diff_out) {   // the true residual (the mixed solver's outer step does not need it)
+    RC(mult_impl(op, q, xq, op->mode, false));
+    double diff2;
+    HIPCHK(launch_sub_norm2(pr, q, q, b, n, g_ctx.red, 0, st));
+    RC(fetch(1, &diff2));
+    *diff_out = std::sqrt(diff2 * snorm);
+  }
   return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "CG: not converged") : DWF_OK;
 }
 
@@ -1406,7 +1409,7 @@ static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv
     double idiff = 0;
     int rc = s->inner->kind == 2 ? fcg_solve(s->inner, sf, rf, &inc, &idiff)
              : (s->inner->fast && opf->mode == "DdagD")
-                 ? cg_solve_fast(s->inner, sf, rf, &inc, &idiff)
+                 ? cg_solve_fast(s->inner, sf, rf, &inc, nullptr)
                  : cg_solve(s->inner, sf, rf, &inc, &idiff);
     if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
     s->inner_iters += s->inner->inner_iters;
@@ -1416,12 +1419,10 @@ static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv
     HIPCHK(launch_sub_norm2(pr, r, r, b, n, g_ctx.red, 0, st));
     RC(fetch(1, &r2));
   }
-  RC(mult_impl(op, r, xq, op->mode, false));
-  double diff2;
-  HIPCHK(launch_sub_norm2(pr, r, r, b, n, g_ctx.red, 0, st));
-  RC(fetch(1, &diff2));
+  // r2 = |b - A xq|^2 of the final xq (computed by the last step, as the
+  // separate residual evaluation after the loop would)
   *nconv_out = nconv;
-  *diff_out = std::sqrt(diff2 * bnorm2_inv);
+  *diff_out = std::sqrt(r2 * bnorm2_inv);
   return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "ASolver_Richardson: not converged") : DWF_OK;
 }
 

@@ -42,6 +42,8 @@ struct Epi {
   // with X: xcoef = xnum[0] / xden[0], read by the kernel from device memory
   const double* xnum = nullptr;
   const double* xden = nullptr;
+  // out == nullptr (Ls = 16, with out2): out is formed for out2 and the norm
+  // but not stored.
   // CG updates fused into the epilogue (Ls = 16 only), a = unum[0] / uden[0]:
   // K_SITE with X: upd += a X (besides out, out2);  hop without out2:
   // upd -= a out, written instead of out, with nrm_* taken of the new upd

@@ -320,34 +320,11 @@ __device__ __forceinline__ void pk_compute(float (*acc)[24], const f16x2_t (&w)[
     reconstruct<DIR>(acc[v], a1, a2);
   }
 }
-#ifndef DWF_HOP_PIPE16
-#define DWF_HOP_PIPE16 0
-#endif
 template <bool DAG>
 __device__ __forceinline__ void hop_all_pk(float (*acc)[24], const __half* __restrict__ in,
                                            const unsigned* shP, const int* shN, int lsite,
                                            int s0, int spb) {
   f16x2_t wa[24];
-#if DWF_HOP_PIPE16
-  // the next direction's loads are in flight while the current one is computed
-  f16x2_t wb[24];
-  pk_load<0, DAG>(wa, in, shN, lsite, s0, spb);
-  pk_load<1, DAG>(wb, in, shN, lsite, s0, spb);
-  pk_compute<0>(acc, wa, shP, lsite);
-  pk_load<2, DAG>(wa, in, shN, lsite, s0, spb);
-  pk_compute<1>(acc, wb, shP, lsite);
-  pk_load<3, DAG>(wb, in, shN, lsite, s0, spb);
-  pk_compute<2>(acc, wa, shP, lsite);
-  pk_load<4, DAG>(wa, in, shN, lsite, s0, spb);
-  pk_compute<3>(acc, wb, shP, lsite);
-  pk_load<5, DAG>(wb, in, shN, lsite, s0, spb);
-  pk_compute<4>(acc, wa, shP, lsite);
-  pk_load<6, DAG>(wa, in, shN, lsite, s0, spb);
-  pk_compute<5>(acc, wb, shP, lsite);
-  pk_load<7, DAG>(wb, in, shN, lsite, s0, spb);
-  pk_compute<6>(acc, wa, shP, lsite);
-  pk_compute<7>(acc, wb, shP, lsite);
-#else
   pk_load<0, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<0>(acc, wa, shP, lsite);
   pk_load<1, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<1>(acc, wa, shP, lsite);
   pk_load<2, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<2>(acc, wa, shP, lsite);
@@ -356,7 +333,6 @@ __device__ __forceinline__ void hop_all_pk(float (*acc)[24], const __half* __res
   pk_load<5, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<5>(acc, wa, shP, lsite);
   pk_load<6, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<6>(acc, wa, shP, lsite);
   pk_load<7, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<7>(acc, wa, shP, lsite);
-#endif
 }
 
 template <bool DAG, typename T, int VEC, typename C>
@@ -780,8 +756,10 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             Q4<T>::stn(upd + ob, ya);
             Q4<T>::stn(upd + ob + 12 * Ls, yb);
           } else {
-            Q4<T>::stn(out + ob, o1);
-            Q4<T>::stn(out + ob + 12 * Ls, o2);
+            if (out) {   // out == nullptr: only out2 and the norm are wanted
+              Q4<T>::stn(out + ob, o1);
+              Q4<T>::stn(out + ob + 12 * Ls, o2);
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
               nacc += (double)o1[r] * (double)o1[r] + (double)o2[r] * (double)o2[r];
@@ -1137,10 +1115,11 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
     const size_t epi = mat ? (size_t)sp * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
-  // the fused CG updates exist in the matrix-core epilogue only
-  if (e.upd && !(mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap &&
-                 (op == K_SITE ? addx : !out2)))
-    return hipErrorInvalidValue;
+  // the fused CG updates, and an output that is not stored, exist in the
+  // matrix-core epilogue only
+  const bool mfma_path = mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap;
+  if (e.upd && !(mfma_path && (op == K_SITE ? addx : !out2))) return hipErrorInvalidValue;
+  if (!out && !e.upd && !(mfma_path && out2)) return hipErrorInvalidValue;
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
     const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
