@@ -1158,72 +1158,49 @@ static int cg_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv_out, do
   return nconv < 0 ? fail(DWF_ERR_NOT_CONVERGED, "ASolver_CG: not converged") : DWF_OK;
 }
 
-// CG on op's DdagD from a zero guess without host round trips inside the
-// iteration: <p, A p> = |D p|^2 comes from the epilogue of the last hop of D,
+// The sync-free CG iteration on op's DdagD (fields x, r, p, q, p2 in op's
+// precision): <p, A p> = |D p|^2 comes from the epilogue of the last hop of D,
 // the coefficients stay in device memory (ping-pong slots), and |r|^2 of
-// iteration i is read back while iteration i + 1 already runs.  The
-// iteration that passes the test is followed by one more (never fewer
-// iterations than the test asks for).  Used as the inner solver of the
-// mixed-precision Richardson, where only the result matters.
-static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
-                         double* diff_out) {
-  dwf_fopr op = s->op;
-  // Non-spin-scalar Mp: the update p = r + beta p runs as a site launch that
-  // also writes Mp p, the first factor of the next D p, into q (free between
-  // the x/r update and the next operator); p alternates between two fields.
-  // At Ls = 16 the r update moves into the last hop (fused matrix-core
-  // epilogue, DdagD p is never stored) and the x update into that site launch,
-  // or, with Mp folded into the hops, into one BLAS pass with the p update.
-  const bool pre = !op->mp_scalar;
-  const bool fused = op->Ls == 16;
-  for (int i = 0; i < (pre ? 5 : 4); ++i) RC(solver_field(op, &s->f[i]));
-  void *x = s->f[0], *r = s->f[1], *p = s->f[2], *q = s->f[3], *p2 = s->f[4];
-  const int pr = op->prec;
-  const size_t n = op->nelem;
-  hipStream_t st = g_ctx.stream;
-  double* res = g_ctx.red.result;
-  const int RA = 8, RB = 9, PAP = 10;
-  double sr;
-  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
-  RC(fetch(1, &sr));
-  if (!(sr > 0)) {
-    HIPCHK(launch_set_zero(pr, xq, n, st));
-    *nconv_out = 0;
-    if (diff_out) *diff_out = 0;
-    s->inner_iters = 0;
-    return DWF_OK;
-  }
-  const double snorm = 1.0 / sr;
-  HIPCHK(launch_copy(pr, r, b, n, st));
-  HIPCHK(launch_copy(pr, p, b, n, st));
-  HIPCHK(launch_set_zero(pr, x, n, st));
-  HIPCHK(launch_norm2(pr, r, n, g_ctx.red, RA, st));
-  if (pre) RC(run_site(op, q, p, M_PRE));
+// iteration i is read by the host while iteration i + 1 already runs.
+// Non-spin-scalar Mp: the update p = r + beta p runs as a site launch that
+// also writes Mp p, the first factor of the next D p, into q (free between
+// the x/r update and the next operator); p alternates between two fields.
+// At Ls = 16 the r update moves into the last hop (fused matrix-core
+// epilogue, DdagD p is never stored) and the x update into that site launch,
+// or, with Mp folded into the hops, into one BLAS pass with the p update.
+struct FastCG {
+  dwf_fopr op = nullptr;
+  bool pre = false, fused = false;
+  void *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *p2 = nullptr;
+  double* res = nullptr;
+  int cur = 8, nxt = 9;
+  static constexpr int PAP = 10;
   // |r|^2 of iteration i reaches the host in ring[i & 1]: written by the
   // finishing reduction kernel itself (fused path) or copied
-  double* ring = fused ? g_ctx.host_ring : g_ctx.host_result + 32;
-  // The fused path needs no event: the host polls its slot, set to -1 before
-  // the iteration is queued (|r|^2 >= 0), and watches the stream for errors.
-  auto wait_ring = [&](int slot) -> int {
-    if (!fused) {
-      HIPCHK(hipEventSynchronize(g_ctx.ring_ev[slot]));
-      return DWF_OK;
-    }
-    volatile double* v = ring + slot;
-    for (unsigned spin = 1;; ++spin) {
-      if (!(*v < 0)) return DWF_OK;
-      if ((spin & 1023) == 0) {
-        const hipError_t q = hipStreamQuery(st);
-        if (q == hipSuccess) {
-          if (!(*v < 0)) return DWF_OK;
-          return fail(DWF_ERR_HIP, "CG: residual norm not delivered");
-        }
-        if (q != hipErrorNotReady) return fail(DWF_ERR_HIP, hipGetErrorString(q));
-      }
-    }
-  };
-  int cur = RA, nxt = RB, nconv = -1, iter = 0;
-  for (; iter < s->niter; ++iter) {
+  double* ring = nullptr;
+
+  int init(dwf_solver_s* s) {
+    op = s->op;
+    pre = !op->mp_scalar;
+    fused = op->Ls == 16;
+    for (int i = 0; i < (pre ? 5 : 4); ++i) RC(solver_field(op, &s->f[i]));
+    x = s->f[0]; r = s->f[1]; p = s->f[2]; q = s->f[3]; p2 = s->f[4];
+    res = g_ctx.red.result;
+    cur = 8; nxt = 9;
+    ring = fused ? g_ctx.host_ring : g_ctx.host_result + 32;
+    return DWF_OK;
+  }
+  // r (|r|^2 in res[cur]) and p set: prepare the first step
+  int start() {
+    if (pre) RC(run_site(op, q, p, M_PRE));
+    return DWF_OK;
+  }
+  // One iteration: r -= alpha A p (|r|^2 -> res[nxt], ring[iter & 1]),
+  // x += alpha p, p = r + beta p.  Swaps cur and nxt.
+  int step(int iter) {
+    const int pr = op->prec;
+    const size_t n = op->nelem;
+    hipStream_t st = g_ctx.stream;
     if (fused) ring[iter & 1] = -1.0;
     if (fused && !pre) {
       RC(op_DdagD_cg(op, p, p, res + PAP, r, res + cur, res + nxt, ring + (iter & 1)));
@@ -1266,9 +1243,65 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
       HIPCHK(hipEventRecord(g_ctx.ring_ev[iter & 1], st));
     }
     std::swap(cur, nxt);
+    return DWF_OK;
+  }
+  // wait for ring[slot]: the fused path polls its slot (set to -1 before the
+  // iteration is queued, |r|^2 >= 0) and watches the stream for errors; the
+  // other path waits for the event after the copy
+  int wait(int slot) {
+    if (!fused) {
+      HIPCHK(hipEventSynchronize(g_ctx.ring_ev[slot]));
+      return DWF_OK;
+    }
+    volatile double* v = ring + slot;
+    for (unsigned spin = 1;; ++spin) {
+      if (!(*v < 0)) return DWF_OK;
+      if ((spin & 1023) == 0) {
+        const hipError_t qs = hipStreamQuery(g_ctx.stream);
+        if (qs == hipSuccess) {
+          if (!(*v < 0)) return DWF_OK;
+          return fail(DWF_ERR_HIP, "CG: residual norm not delivered");
+        }
+        if (qs != hipErrorNotReady) return fail(DWF_ERR_HIP, hipGetErrorString(qs));
+      }
+    }
+  }
+};
+
+// CG on op's DdagD from a zero guess with the sync-free iteration.  The
+// iteration that passes the test is followed by one more (never fewer
+// iterations than the test asks for).  Used as the inner solver of the
+// mixed-precision Richardson, where only the result matters.
+static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
+                         double* diff_out) {
+  FastCG c;
+  RC(c.init(s));
+  dwf_fopr op = s->op;
+  const int pr = op->prec;
+  const size_t n = op->nelem;
+  hipStream_t st = g_ctx.stream;
+  double sr;
+  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &sr));
+  if (!(sr > 0)) {
+    HIPCHK(launch_set_zero(pr, xq, n, st));
+    *nconv_out = 0;
+    if (diff_out) *diff_out = 0;
+    s->inner_iters = 0;
+    return DWF_OK;
+  }
+  const double snorm = 1.0 / sr;
+  HIPCHK(launch_copy(pr, c.r, b, n, st));
+  HIPCHK(launch_copy(pr, c.p, b, n, st));
+  HIPCHK(launch_set_zero(pr, c.x, n, st));
+  HIPCHK(launch_norm2(pr, c.r, n, g_ctx.red, c.cur, st));
+  RC(c.start());
+  int nconv = -1, iter = 0;
+  for (; iter < s->niter; ++iter) {
+    RC(c.step(iter));
     if (iter >= 1) {
-      RC(wait_ring((iter - 1) & 1));
-      if (ring[(iter - 1) & 1] * snorm < s->stop) {
+      RC(c.wait((iter - 1) & 1));
+      if (c.ring[(iter - 1) & 1] * snorm < s->stop) {
         nconv = iter;   // the test passed after iteration iter - 1 (0-based)
         ++iter;         // iterations run: iter + 1
         break;
@@ -1276,16 +1309,16 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
     }
   }
   if (nconv < 0 && iter == s->niter && iter >= 1) {   // the last iteration's test
-    RC(wait_ring((iter - 1) & 1));
-    if (ring[(iter - 1) & 1] * snorm < s->stop) nconv = iter;
+    RC(c.wait((iter - 1) & 1));
+    if (c.ring[(iter - 1) & 1] * snorm < s->stop) nconv = iter;
   }
-  HIPCHK(launch_copy(pr, xq, x, n, st));
+  HIPCHK(launch_copy(pr, xq, c.x, n, st));
   *nconv_out = nconv;
   s->inner_iters = iter;
   if (diff_out) {   // the true residual (the mixed solver's outer step does not need it)
-    RC(mult_impl(op, q, xq, op->mode, false));
+    RC(mult_impl(op, c.q, xq, op->mode, false));
     double diff2;
-    HIPCHK(launch_sub_norm2(pr, q, q, b, n, g_ctx.red, 0, st));
+    HIPCHK(launch_sub_norm2(pr, c.q, c.q, b, n, g_ctx.red, 0, st));
     RC(fetch(1, &diff2));
     *diff_out = std::sqrt(diff2 * snorm);
   }
