@@ -468,8 +468,11 @@ template <> struct KCfg<float> {
   static constexpr int THREADS = DWF_HOP_THREADS32;
   static constexpr int WAVES = DWF_WAVES_F32;
 };
+#ifndef DWF_VEC_F64
+#define DWF_VEC_F64 1
+#endif
 template <> struct KCfg<double> {
-  static constexpr int VEC = 1;
+  static constexpr int VEC = DWF_VEC_F64;
   static constexpr int THREADS = DWF_HOP_THREADS;
   static constexpr int WAVES = DWF_WAVES_F64;
 };
@@ -1984,11 +1987,68 @@ k_shift_update(double* const* __restrict__ x, double* const* __restrict__ p,
                   pp + j);
 }
 
+// The same update with the shifts looped inside each thread (m <= kShiftMax):
+// r is read once per element instead of once per shift (4 passes per shift
+// + 1 instead of 5 per shift).  The partial sums of |p_j|^2 are laid out as in
+// k_shift_update (region j - 1 of kRedBlocks per shift).  n must be even.
+constexpr int kShiftMax = 32;
+__global__ void __launch_bounds__(256)
+k_shift_update_all(double* const* __restrict__ x, double* const* __restrict__ p,
+                   const double* __restrict__ r, const double* __restrict__ coef, int m,
+                   size_t n, double* partial) {
+  double v[kShiftMax];
+#pragma unroll
+  for (int j = 0; j < kShiftMax; ++j) v[j] = 0;
+  PACK_LOOP(double, n) {
+    double rv[2];
+    Vec<double>::ld(r + ip * 2, rv);
+#pragma unroll
+    for (int j = 1; j < kShiftMax; ++j) {
+      if (j < m) {
+        double* xj = x[j];
+        double* pj = p[j];
+        const double c0 = coef[3 * j], c1 = coef[3 * j + 1], c2 = coef[3 * j + 2];
+        double xv[2], pv[2];
+        Vec<double>::ld(xj + ip * 2, xv);
+        Vec<double>::ld(pj + ip * 2, pv);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          xv[k] += c0 * pv[k];
+          pv[k] = c1 * pv[k] + c2 * rv[k];
+          v[j] += pv[k] * pv[k];
+        }
+        Vec<double>::st(xj + ip * 2, xv);
+        Vec<double>::st(pj + ip * 2, pv);
+      }
+    }
+  }
+  // (n is even: no tail; 256-thread blocks: 4 waves)
+  __shared__ double sh[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 1; j < kShiftMax; ++j) {
+    if (j < m) {
+      const double t = wave_sum(v[j]);
+      if (lane == 0) sh[wid] = t;
+      __syncthreads();
+      if (threadIdx.x == 0)
+        partial[(size_t)(j - 1) * kRedBlocks + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+      __syncthreads();
+    }
+  }
+}
+
 hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
                                const double* coef, int m, size_t n, double* partial,
                                unsigned* counters, double* pp, hipStream_t st) {
   if (m <= 1) return hipSuccess;
   dim3 g = red_grid(n);
+  if (m <= kShiftMax && n % 2 == 0 && kRedTwoKernels) {
+    hipLaunchKernelGGL(k_shift_update_all, g, dim3(256), 0, st, (double* const*)x,
+                       (double* const*)p, (const double*)r, coef, m, n, partial);
+    red_finish<1>(partial, g.x, pp + 1, st, m - 1, kRedBlocks, 1);
+    return hipGetLastError();
+  }
   g.y = m - 1;
   hipLaunchKernelGGL(k_shift_update, g, dim3(256), 0, st, (double* const*)x, (double* const*)p,
                      (const double*)r, coef, n, partial, counters, pp);
