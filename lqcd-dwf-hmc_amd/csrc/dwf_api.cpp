@@ -1272,8 +1272,9 @@ struct FastCG {
 // iteration that passes the test is followed by one more (never fewer
 // iterations than the test asks for).  Used as the inner solver of the
 // mixed-precision Richardson, where only the result matters.
+// b_scratch: b may be overwritten (it then serves as the residual field).
 static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_out,
-                         double* diff_out) {
+                         double* diff_out, bool b_scratch = false) {
   FastCG c;
   RC(c.init(s));
   dwf_fopr op = s->op;
@@ -1281,8 +1282,8 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
   const size_t n = op->nelem;
   hipStream_t st = g_ctx.stream;
   double sr;
-  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, 0, st));
-  RC(fetch(1, &sr));
+  HIPCHK(launch_norm2(pr, b, n, g_ctx.red, c.cur, st));   // = |r|^2 of the start
+  RC(fetch(1, &sr, c.cur));
   if (!(sr > 0)) {
     HIPCHK(launch_set_zero(pr, xq, n, st));
     *nconv_out = 0;
@@ -1291,10 +1292,12 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
     return DWF_OK;
   }
   const double snorm = 1.0 / sr;
-  HIPCHK(launch_copy(pr, c.r, b, n, st));
+  if (b_scratch && !diff_out)
+    c.r = const_cast<void*>(b);
+  else
+    HIPCHK(launch_copy(pr, c.r, b, n, st));
   HIPCHK(launch_copy(pr, c.p, b, n, st));
   HIPCHK(launch_set_zero(pr, c.x, n, st));
-  HIPCHK(launch_norm2(pr, c.r, n, g_ctx.red, c.cur, st));
   RC(c.start());
   int nconv = -1, iter = 0;
   for (; iter < s->niter; ++iter) {
@@ -1436,18 +1439,29 @@ static int richardson_solve(dwf_solver_s* s, void* xq, const void* b, int* nconv
       break;
     }
     const double normalization = std::sqrt(r2);
-    HIPCHK(launch_scal(pr, r, 1.0 / normalization, n, st));
-    HIPCHK(launch_convert(opf->prec, rf, pr, r, n, st));
+    // r / |r| in the inner precision (one pass; the same values as scaling in
+    // place and converting)
+    if (pr == P64) {
+      HIPCHK(launch_convert_scaled(opf->prec, rf, pr, r, 1.0 / normalization, n, st));
+    } else {
+      HIPCHK(launch_scal(pr, r, 1.0 / normalization, n, st));
+      HIPCHK(launch_convert(opf->prec, rf, pr, r, n, st));
+    }
     int inc = 0;
     double idiff = 0;
+    // the fast inner CG may use rf as its residual field (rf is rebuilt each step)
     int rc = s->inner->kind == 2 ? fcg_solve(s->inner, sf, rf, &inc, &idiff)
              : (s->inner->fast && opf->mode == "DdagD")
-                 ? cg_solve_fast(s->inner, sf, rf, &inc, nullptr)
+                 ? cg_solve_fast(s->inner, sf, rf, &inc, nullptr, true)
                  : cg_solve(s->inner, sf, rf, &inc, &idiff);
     if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
     s->inner_iters += s->inner->inner_iters;
-    HIPCHK(launch_convert(pr, sv, opf->prec, sf, n, st));
-    HIPCHK(launch_axpy(pr, xq, -normalization, sv, n, st));
+    if (pr == P64) {   // xq -= |r| s in one pass
+      HIPCHK(launch_axpy_from(pr, xq, -normalization, opf->prec, sf, n, st));
+    } else {
+      HIPCHK(launch_convert(pr, sv, opf->prec, sf, n, st));
+      HIPCHK(launch_axpy(pr, xq, -normalization, sv, n, st));
+    }
     RC(mult_impl(op, r, xq, op->mode, false));
     HIPCHK(launch_sub_norm2(pr, r, r, b, n, g_ctx.red, 0, st));
     RC(fetch(1, &r2));

@@ -73,6 +73,12 @@ hipError_t launch_aypx(int prec, double a, void* y, const void* x, size_t n, hip
 hipError_t launch_scal(int prec, void* x, double a, size_t n, hipStream_t st);
 hipError_t launch_convert(int prec_out, void* y, int prec_in, const void* x, size_t n,
                           hipStream_t st);
+// y = a x (x double, y any storage type; one rounding)
+hipError_t launch_convert_scaled(int prec_out, void* y, int prec_in, const void* x, double a,
+                                 size_t n, hipStream_t st);
+// y += a x (y double, x any storage type)
+hipError_t launch_axpy_from(int prec_y, void* y, double a, int prec_x, const void* x, size_t n,
+                            hipStream_t st);
 
 // Reductions: write to result[0..] (device), using the scratch partial/counter.
 struct RedScratch {

@@ -1497,6 +1497,50 @@ static hipError_t convert_from(void* y, int prec_in, const void* x, size_t n, hi
   }
   return hipGetLastError();
 }
+// y = a x across storage types (the product in double, one rounding to y's type)
+template <typename TO, typename TI>
+__global__ void __launch_bounds__(256) k_convert_scaled(TO* __restrict__ y, const TI* __restrict__ x,
+                                                        double a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    St<TO>::st(y + i, (typename St<TO>::C)(a * (double)St<TI>::ld(x + i)));
+}
+// y += a x with x in another storage type (double y)
+template <typename TI>
+__global__ void __launch_bounds__(256) k_axpy_from(double* __restrict__ y, double a,
+                                                   const TI* __restrict__ x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] += a * (double)St<TI>::ld(x + i);
+}
+hipError_t launch_convert_scaled(int prec_out, void* y, int prec_in, const void* x, double a,
+                                 size_t n, hipStream_t st) {
+  if (prec_in != P64) return hipErrorInvalidValue;
+  const dim3 g = grid_for(n), b(256);
+  const double* xd = (const double*)x;
+  switch (prec_out) {
+    case P64: hipLaunchKernelGGL((k_convert_scaled<double, double>), g, b, 0, st, (double*)y, xd, a, n); break;
+    case P32: hipLaunchKernelGGL((k_convert_scaled<float, double>), g, b, 0, st, (float*)y, xd, a, n); break;
+    case P16: hipLaunchKernelGGL((k_convert_scaled<__half, double>), g, b, 0, st, (__half*)y, xd, a, n); break;
+    case PBF16: hipLaunchKernelGGL((k_convert_scaled<__hip_bfloat16, double>), g, b, 0, st, (__hip_bfloat16*)y, xd, a, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_axpy_from(int prec_y, void* y, double a, int prec_x, const void* x, size_t n,
+                            hipStream_t st) {
+  if (prec_y != P64) return hipErrorInvalidValue;
+  const dim3 g = grid_for(n), b(256);
+  switch (prec_x) {
+    case P64: hipLaunchKernelGGL((k_axpy_from<double>), g, b, 0, st, (double*)y, a, (const double*)x, n); break;
+    case P32: hipLaunchKernelGGL((k_axpy_from<float>), g, b, 0, st, (double*)y, a, (const float*)x, n); break;
+    case P16: hipLaunchKernelGGL((k_axpy_from<__half>), g, b, 0, st, (double*)y, a, (const __half*)x, n); break;
+    case PBF16: hipLaunchKernelGGL((k_axpy_from<__hip_bfloat16>), g, b, 0, st, (double*)y, a, (const __hip_bfloat16*)x, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_convert(int prec_out, void* y, int prec_in, const void* x, size_t n,
                           hipStream_t st) {
   switch (prec_out) {
