@@ -9,7 +9,7 @@ rho = 0.1 on the GPU; Gaussian complex source on the even sites; target
 
 One timed step = one solve by FP64 Richardson (the reference's
 Fprop_alt_Standard_eo_Richardson outer solver) around a sync-free CG on the
-FP16-storage operator with inner |r|^2/|b|^2 < 1e-4 (the fastest measured
+FP16-storage operator with inner |r|^2/|b|^2 < 2e-4 (the fastest measured
 configuration).  Also measured, outside the timed steps: the plain FP64 CG
 solve of the same system, the reference's scheme (FP32 inner CG to 1e-8), the
 Richardson with the learned polynomial preconditioner (training time apart),
@@ -40,7 +40,7 @@ MQ, M0, B, CC = 0.01, 1.6, 1.0, 0.0
 EPS, RHO, NSMEAR = 0.3, 0.1, 2
 TOL2 = 1e-20          # |r|^2 / |b|^2  <->  |r|/|b| < 1e-10
 INNER_TOL2 = 1e-8     # fprop_alt_Standard_eo_Richardson-tmpl.h:60 (inner criterion)
-INNER_TOL2_FAST = 1e-4   # inner criterion of the timed solver (measured fastest)
+INNER_TOL2_FAST = 2e-4   # inner criterion of the timed solver (fastest over 5 sources)
 
 
 def log(*a):
@@ -299,7 +299,7 @@ def main():
             "workload": "mixed-precision solve DdagD x = b to |r|/|b|<1e-10, 16^3x32, Ls=16, "
                         "mq=0.01, M0=1.6, Shamir, exp(0.3 iH) links + 2 stout(0.1) steps",
             "lattice": list(DIMS), "Ls": LS,
-            "step": "one FP64-Richardson / FP16-storage-CG solve (inner |r|^2/|b|^2 < 1e-4)",
+            "step": "one FP64-Richardson / FP16-storage-CG solve (inner |r|^2/|b|^2 < 2e-4)",
             "value": "FP64 D_eo algorithmic HBM GB/s",
         },
         "time_to_solution_s": {

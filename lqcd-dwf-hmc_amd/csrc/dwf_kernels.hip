@@ -549,18 +549,61 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     if constexpr (PK) {
       unsigned* shP = reinterpret_cast<unsigned*>(shA);
       const unsigned short* Lh = reinterpret_cast<const unsigned short*>(L);
-      for (int i = tid; i < 72 * spb; i += blockDim.x) {
-        const int row = i / spb, j = i - row * spb;   // row = dir * 9 + entry
+      // two sites per 4-byte load (site0 and spb are even; Vh even keeps the
+      // rows 4-byte aligned, else the sites are loaded one by one)
+      const int hp = spb >> 1;
+      const bool pair_ok = (Vh & 1) == 0;
+      for (int i = tid; i < 72 * hp; i += blockDim.x) {
+        const int row = i / hp, jp = i - row * hp;   // row = dir * 9 + entry
         const int dir = row / 9, ab = row - dir * 9;
-        unsigned x = 0, y = 0;
-        if (j < nsite) {
-          const unsigned ur = __builtin_nontemporal_load(Lh + (size_t)(dir * 18 + 2 * ab) * Vh + j);
-          const unsigned ui = __builtin_nontemporal_load(Lh + (size_t)(dir * 18 + 2 * ab + 1) * Vh + j);
-          x = ur | ((ui ^ 0x8000u) << 16);   // (ur, -ui)
-          y = ui | (ur << 16);               // (ui, ur)
+        const int j = 2 * jp;
+        const unsigned short* pr = Lh + (size_t)(dir * 18 + 2 * ab) * Vh + j;
+        const unsigned short* pi = pr + Vh;
+        unsigned r2 = 0, i2 = 0;
+        if (pair_ok && j + 1 < nsite) {
+          r2 = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(pr));
+          i2 = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(pi));
+        } else {
+          if (j < nsite) {
+            r2 = __builtin_nontemporal_load(pr);
+            i2 = __builtin_nontemporal_load(pi);
+          }
+          if (j + 1 < nsite) {
+            r2 |= (unsigned)__builtin_nontemporal_load(pr + 1) << 16;
+            i2 |= (unsigned)__builtin_nontemporal_load(pi + 1) << 16;
+          }
         }
-        shP[j * kPkStride + dir * 20 + 2 * ab] = x;
-        shP[j * kPkStride + dir * 20 + 2 * ab + 1] = y;
+        const unsigned ur0 = r2 & 0xffffu, ur1 = r2 >> 16, ui0 = i2 & 0xffffu, ui1 = i2 >> 16;
+        unsigned* d0 = shP + j * kPkStride + dir * 20 + 2 * ab;
+        d0[0] = ur0 | ((ui0 ^ 0x8000u) << 16);   // (ur, -ui)
+        d0[1] = ui0 | (ur0 << 16);               // (ui, ur)
+        d0[kPkStride] = ur1 | ((ui1 ^ 0x8000u) << 16);
+        d0[kPkStride + 1] = ui1 | (ur1 << 16);
+      }
+    } else if constexpr (sizeof(T) >= 4) {
+      // 16-byte loads of W consecutive sites when the rows stay aligned
+      constexpr int W = 16 / sizeof(T);
+      typedef T tw_t __attribute__((ext_vector_type(W)));
+      if ((Vh % W) == 0 && (spb % W) == 0) {
+        const int sw = spb / W;
+        for (int i = tid; i < 144 * sw; i += blockDim.x) {
+          const int row = i / sw, j = (i - row * sw) * W;
+          const T* src = L + (size_t)row * Vh + j;
+          C* d = shL + row * spb + j;
+          if (j + W <= nsite) {
+            const tw_t v = __builtin_nontemporal_load(reinterpret_cast<const tw_t*>(src));
+#pragma unroll
+            for (int k = 0; k < W; ++k) d[k] = v[k];
+          } else {
+#pragma unroll
+            for (int k = 0; k < W; ++k) d[k] = (j + k < nsite) ? S::ldn(src + k) : C(0);
+          }
+        }
+      } else {
+        for (int i = tid; i < 144 * spb; i += blockDim.x) {
+          const int row = i / spb, j = i - row * spb;
+          shL[i] = (j < nsite) ? S::ldn(L + (size_t)row * Vh + j) : C(0);
+        }
       }
     } else {
       for (int i = tid; i < 144 * spb; i += blockDim.x) {
