@@ -191,12 +191,34 @@ def main():
         f32.mult(v32, b32, "Deo")
     ev.record(e1, st)
     deo32_ms = ev.elapsed_ms(e0, e1) / a.dslash_reps
+    v16, b16 = f16.new_field(), f16.new_field()
+    b16.convert_precision_from(b)
+    ev.record(e0, st)
+    for _ in range(a.dslash_reps):
+        f16.mult(v16, b16, "Deo")
+    ev.record(e1, st)
+    deo16_ms = ev.elapsed_ms(e0, e1) / a.dslash_reps
     ev.record(e0, st)
     for _ in range(20):
         f64.mult(v, b)
     ev.record(e1, st)
     ddd_ms = ev.elapsed_ms(e0, e1) / 20
     log(f"D_eo FP32: {deo32_ms * 1e3:.1f} us; DdagD FP64: {ddd_ms * 1e3:.1f} us")
+    # error of the reduced-precision operators against FP64 on the same input:
+    # |D^prec w - D^64 w| / |D^64 w| for D_eo and DdagD (w = the solve's source)
+    op_err = {}
+    for name, fp in (("f32", f32), ("f16", f16)):
+        for mode in ("Deo", "DdagD"):
+            wl, vl, vd = fp.new_field(), fp.new_field(), f64.new_field()
+            wl.convert_precision_from(b)
+            m = None if mode == "DdagD" else mode   # the operators' mode is DdagD
+            fp.mult(vl, wl, m)
+            f64.mult(v, b, m)
+            vd.convert_precision_from(vl)
+            ref2 = v.norm2()
+            vd.axpy(-1.0, v)
+            op_err[f"{mode}_{name}"] = float((vd.norm2() / ref2) ** 0.5)
+    log(f"operator error vs FP64: {op_err}")
 
     # ---- timed steps: mixed-precision solves --------------------------------
     # FP64 Richardson (the reference's outer solver) around a sync-free CG on the
@@ -323,6 +345,15 @@ def main():
         "multishift_16_shifts_fp64": {"time_s": round(t_ms, 4), "iterations": n_ms,
                                       "ms_per_iteration": round(t_ms / max(n_ms, 1) * 1e3, 3),
                                       "max_rel_residual": d_ms},
+        "operator_rel_error_vs_fp64": {k: float(f"{e:.3e}") for k, e in op_err.items()},
+        # the hop the timed solve spends its time in: FP16 storage D_eo, algorithmic
+        # bytes (16-bit spinors and links) per launch over its HIP-event time
+        "inner_hop_fp16_deo": {
+            "us": round(deo16_ms * 1e3, 2),
+            "bytes_per_launch": deo_bytes(Vh, LS, 2),
+            "achieved_GBs": round(deo_bytes(Vh, LS, 2) / (deo16_ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(deo_bytes(Vh, LS, 2) / (deo16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
         "kernels_us": {"deo_f64": round(deo_ms * 1e3, 2), "deo_f32": round(deo32_ms * 1e3, 2),
                        "ddagd_f64": round(ddd_ms * 1e3, 2)},
         "roofline": {

@@ -513,6 +513,7 @@ static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm,
 // the hops); |D p|^2 -> *pap; then r -= (*rr / *pap) DdagD p, and |r|^2 of the
 // new r -> *rr_new (and *rr_host, coherent host memory).  DdagD p itself is
 // not stored.
+// |D p|^2 is added up by the last block of the third hop (no finishing launch).
 static int op_DdagD_cg(dwf_fopr_s* op, const void* p, const void* mpw, double* pap, void* r,
                        const double* rr, double* rr_new, double* rr_host) {
   void *t1, *u2;
@@ -530,9 +531,20 @@ static int op_DdagD_cg(dwf_fopr_s* op, const void* p, const void* mpw, double* p
   e.F = mat(op, M_DIT);
   e.nrm_partial = op->hop_partial;
   e.nrm_result = pap;
+  unsigned nb2 = 0;
+  e.nrm_defer = true;   // added up by the next hop (256-thread blocks; the launch checks)
+  e.nblocks_out = &nb2;
   HIPCHK(launch_dwf(op->prec, K_HOP, nullptr, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
                     g_ctx.stream));
-  RC(run_hop(op, true, 1, t1, u2, M_G1));
+  {
+    Epi h;   // first hop of D^dag; its last block adds up |D p|^2
+    h.E = mat(op, M_G1);
+    h.fin_partial = op->hop_partial;
+    h.fin_n = nb2;
+    h.fin_result = pap;
+    HIPCHK(launch_dwf(op->prec, K_HOPDAG, t1, u2, op->links, op->nbr, 1, op->Vh, op->Ls, h,
+                      g_ctx.stream));
+  }
   Epi g;
   g.E = mat(op, M_G2);
   g.X = u2;

@@ -50,6 +50,15 @@ struct Epi {
   void* upd = nullptr;
   const double* unum = nullptr;
   const double* uden = nullptr;
+  // nrm_defer: leave the block partials of the norm for a later launch to add
+  // (no finishing kernel); nblocks_out receives the launch's block count
+  bool nrm_defer = false;
+  unsigned* nblocks_out = nullptr;
+  // add fin_n partials into *fin_result in the launch's last block (the
+  // finishing of an earlier launch's deferred norm)
+  const double* fin_partial = nullptr;
+  unsigned fin_n = 0;
+  double* fin_result = nullptr;
 };
 
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,

@@ -32,6 +32,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Sum of n partials by one 256-thread block, in the order k_red_finish uses
+// (thread t adds t, t + 256, ...; then the four wave sums in wave order), so
+// that every block that calls it, and k_red_finish, get the same value.
+__device__ __forceinline__ double block_sum256(const double* __restrict__ p, unsigned n,
+                                               double* sh4) {
+  double w = 0;
+  for (unsigned i = threadIdx.x; i < n; i += 256) w += p[i];
+  const double t = wave_sum(w);
+  if ((threadIdx.x & 63) == 0) sh4[threadIdx.x >> 6] = t;
+  __syncthreads();
+  const double r = sh4[0] + sh4[1] + sh4[2] + sh4[3];
+  __syncthreads();
+  return r;
+}
+
 size_t prec_bytes(int prec) {
   switch (prec) {
     case P64: return 8;
@@ -488,7 +503,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
       const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
       const T* X2, double* __restrict__ nrm, const double* xnum, const double* xden,
-      T* __restrict__ upd, const double* unum, const double* uden) {
+      T* __restrict__ upd, const double* unum, const double* uden,
+      const double* fin_partial, unsigned fin_n, double* fin_result) {
   using S = St<T>;
   using C = typename S::C;
   using VL = Vx<T, VEC>;
@@ -525,6 +541,13 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   const bool xmat = ADDX && MAT && G != nullptr;
 
   const int tid = threadIdx.x;
+  // a reduction left by the previous launch, finished by this launch's last
+  // block (256-thread blocks only: the host checks)
+  if (fin_partial && blockIdx.x == gridDim.x - 1) {
+    __shared__ double fsh[4];
+    const double t = block_sum256(fin_partial, fin_n, fsh);
+    if (tid == 0) fin_result[0] = t;
+  }
   const int lsite = tid / Q;
   const int s0 = (tid - lsite * Q) * VEC;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1088,7 +1111,7 @@ static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* ou
                      (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
                      (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F,
                      (const C*)e.G, (const T*)e.X2, e.nrm_partial, e.xnum, e.xden, (T*)e.upd,
-                     e.unum, e.uden);
+                     e.unum, e.uden, e.fin_partial, e.fin_n, e.fin_result);
 }
 
 // the (op, epilogue) combinations the operator chains use
@@ -1166,6 +1189,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const bool mfma_path = mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap;
   if (e.upd && !(mfma_path && (op == K_SITE ? addx : !out2))) return hipErrorInvalidValue;
   if (!out && !e.upd && !(mfma_path && out2)) return hipErrorInvalidValue;
+  if (e.fin_partial && KCfg<T>::THREADS != 256) return hipErrorInvalidValue;
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
     const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
@@ -1201,7 +1225,8 @@ hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* l
       err = launch_dwf_t<__hip_bfloat16>(op, out, in, links, nbr, parity, Vh, Ls, e, st, &nb);
       break;
   }
-  if (err == hipSuccess && e.nrm_partial) {
+  if (e.nblocks_out) *e.nblocks_out = nb;
+  if (err == hipSuccess && e.nrm_partial && !e.nrm_defer) {
     finish_norm(e.nrm_partial, nb, e.nrm_result, st, e.nrm_host);
     err = hipGetLastError();
   }
