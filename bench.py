@@ -139,8 +139,8 @@ def main():
     ap.add_argument("--no-fp64-cg", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the lattice would shard by 4D domain decomposition; see DESIGN.md).",
+        print("bench.py measures one GPU only: there is no multi-GPU path (the lattice "
+              "would shard by 4D domain decomposition with one halo exchange per hop).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 """Writes the committed fixtures under tests/golden/ from the reference's data files.
 
-Run in the build container only (the reference tree is not on the GPU box):
+Run where the reference tree is present (it is read from /root/reference):
     python tests/golden/make_fixtures.py
 It copies data, never code:
   * conf_04040408.npy      the 4x4x4x8 gauge field of conf_04040408.txt, in file
