@@ -169,7 +169,7 @@ def test_richardson_mixed_precision_matches_oracle():
     assert abs(n - no) <= 1
 
 
-@pytest.mark.parametrize("inner", ["half", "float", "bfloat16"])
+@pytest.mark.parametrize("inner", ["half", "float", "bfloat16", "double"])
 @pytest.mark.parametrize("prm", LS16, ids=["shamir", "mobius"])
 def test_richardson_low_precision_inner_ls16(prm, inner):
     """Mixed solve with the sync-free inner CG at Ls = 16 (matrix-core
@@ -210,6 +210,25 @@ def test_multishift_matches_oracle():
         r = orc.mult("DdagD", x) + s * x - b
         assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
         assert rel(x, xr) < 1e-8
+
+
+def test_multishift_many_shifts():
+    """More shifts than the one-pass update handles (32): the per-shift launch
+    path; every shift meets the stopping condition."""
+    prm = ((4, 4, 2, 4), 4, 0.1, 1.6, 1.0, 0.0, (1, 1, 1, -1))
+    orc, gpu, U = pair(prm)
+    b = lattice.random_spinor5(prm[0], prm[1], 20)
+    sigma = lattice.log_spaced_shifts(36, 1e-2, 10.0)
+    gpu.set_mode("DdagD")
+    fb = gpu.new_field()
+    gpu.convert(fb, b)
+    xs = [gpu.new_field() for _ in sigma]
+    n, d = dwf_amd.AShiftsolver_CG(gpu, 3000, 1e-24).solve(xs, sigma, fb)
+    assert d < 1e-10
+    for s_, fx in zip(sigma, xs):
+        x = gpu.reverse(fx)
+        r = orc.mult("DdagD", x) + s_ * x - b
+        assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
 
 
 def test_stout_smearing_matches_oracle():
