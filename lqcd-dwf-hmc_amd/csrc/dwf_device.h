@@ -132,6 +132,19 @@ template <> struct Pk16<__hip_bfloat16> {
   static __device__ __forceinline__ float hi(unsigned w) { return b_hi(w); }
   static __device__ __forceinline__ unsigned pack(float a, float b) { return b_pack(a, b); }
 };
+// 16-bit outputs are written as 4- or 8-byte pieces of 32-byte rows: they go
+// through L2 as ordinary stores, so that the pieces of a line are merged before
+// the line is written back (non-temporal stores wrote 1.33x the bytes).
+#ifndef DWF_NT_STORE16
+#define DWF_NT_STORE16 0
+#endif
+template <typename R> __device__ __forceinline__ void st16(R* p, R v) {
+#if DWF_NT_STORE16
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
 template <typename T> struct Vx16_2 {
   using R = unsigned;
   static __device__ __forceinline__ R ld(const T* p) { return *reinterpret_cast<const unsigned*>(p); }
@@ -139,7 +152,7 @@ template <typename T> struct Vx16_2 {
     return v == 0 ? Pk16<T>::lo(r) : Pk16<T>::hi(r);
   }
   static __device__ __forceinline__ void stn(T* p, const float* v) {
-    __builtin_nontemporal_store(Pk16<T>::pack(v[0], v[1]), reinterpret_cast<unsigned*>(p));
+    st16(reinterpret_cast<unsigned*>(p), Pk16<T>::pack(v[0], v[1]));
   }
 };
 template <typename T> struct Vx16_4 {
@@ -151,7 +164,7 @@ template <typename T> struct Vx16_4 {
   }
   static __device__ __forceinline__ void stn(T* p, const float* v) {
     R t = {Pk16<T>::pack(v[0], v[1]), Pk16<T>::pack(v[2], v[3])};
-    __builtin_nontemporal_store(t, reinterpret_cast<R*>(p));
+    st16(reinterpret_cast<R*>(p), t);
   }
 };
 template <> struct Vx<__half, 2> : Vx16_2<__half> {};
