@@ -572,36 +572,47 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     if constexpr (PK) {
       unsigned* shP = reinterpret_cast<unsigned*>(shA);
       const unsigned short* Lh = reinterpret_cast<const unsigned short*>(L);
-      // two sites per 4-byte load (site0 and spb are even; Vh even keeps the
-      // rows 4-byte aligned, else the sites are loaded one by one)
-      const int hp = spb >> 1;
-      const bool pair_ok = (Vh & 1) == 0;
-      for (int i = tid; i < 72 * hp; i += blockDim.x) {
-        const int row = i / hp, jp = i - row * hp;   // row = dir * 9 + entry
-        const int dir = row / 9, ab = row - dir * 9;
-        const int j = 2 * jp;
-        const unsigned short* pr = Lh + (size_t)(dir * 18 + 2 * ab) * Vh + j;
-        const unsigned short* pi = pr + Vh;
-        unsigned r2 = 0, i2 = 0;
-        if (pair_ok && j + 1 < nsite) {
-          r2 = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(pr));
-          i2 = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(pi));
-        } else {
-          if (j < nsite) {
-            r2 = __builtin_nontemporal_load(pr);
-            i2 = __builtin_nontemporal_load(pi);
+      // eight sites per 16-byte load when the rows stay 16-byte aligned (Vh and
+      // spb multiples of 8), else one site at a time
+      if ((Vh & 7) == 0 && (spb & 7) == 0) {
+        const int h8 = spb >> 3;
+        for (int i = tid; i < 72 * h8; i += blockDim.x) {
+          const int row = i / h8, j = (i - row * h8) * 8;   // row = dir * 9 + entry
+          const int dir = row / 9, ab = row - dir * 9;
+          const unsigned short* pr = Lh + (size_t)(dir * 18 + 2 * ab) * Vh + j;
+          const unsigned short* pi = pr + Vh;
+          u32x4_t r4 = {0, 0, 0, 0}, i4 = {0, 0, 0, 0};
+          if (j + 8 <= nsite) {
+            r4 = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(pr));
+            i4 = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(pi));
+          } else {
+            for (int m = 0; m < 8; ++m)
+              if (j + m < nsite) {
+                r4[m >> 1] |= (unsigned)__builtin_nontemporal_load(pr + m) << (16 * (m & 1));
+                i4[m >> 1] |= (unsigned)__builtin_nontemporal_load(pi + m) << (16 * (m & 1));
+              }
           }
-          if (j + 1 < nsite) {
-            r2 |= (unsigned)__builtin_nontemporal_load(pr + 1) << 16;
-            i2 |= (unsigned)__builtin_nontemporal_load(pi + 1) << 16;
+          unsigned* d0 = shP + j * kPkStride + dir * 20 + 2 * ab;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            const unsigned ur = (r4[m >> 1] >> (16 * (m & 1))) & 0xffffu;
+            const unsigned ui = (i4[m >> 1] >> (16 * (m & 1))) & 0xffffu;
+            d0[m * kPkStride] = ur | ((ui ^ 0x8000u) << 16);   // (ur, -ui)
+            d0[m * kPkStride + 1] = ui | (ur << 16);           // (ui, ur)
           }
         }
-        const unsigned ur0 = r2 & 0xffffu, ur1 = r2 >> 16, ui0 = i2 & 0xffffu, ui1 = i2 >> 16;
-        unsigned* d0 = shP + j * kPkStride + dir * 20 + 2 * ab;
-        d0[0] = ur0 | ((ui0 ^ 0x8000u) << 16);   // (ur, -ui)
-        d0[1] = ui0 | (ur0 << 16);               // (ui, ur)
-        d0[kPkStride] = ur1 | ((ui1 ^ 0x8000u) << 16);
-        d0[kPkStride + 1] = ui1 | (ur1 << 16);
+      } else {
+        for (int i = tid; i < 72 * spb; i += blockDim.x) {
+          const int row = i / spb, j = i - row * spb;
+          const int dir = row / 9, ab = row - dir * 9;
+          unsigned ur = 0, ui = 0;
+          if (j < nsite) {
+            ur = __builtin_nontemporal_load(Lh + (size_t)(dir * 18 + 2 * ab) * Vh + j);
+            ui = __builtin_nontemporal_load(Lh + (size_t)(dir * 18 + 2 * ab + 1) * Vh + j);
+          }
+          shP[j * kPkStride + dir * 20 + 2 * ab] = ur | ((ui ^ 0x8000u) << 16);
+          shP[j * kPkStride + dir * 20 + 2 * ab + 1] = ui | (ur << 16);
+        }
       }
     } else if constexpr (sizeof(T) >= 4) {
       // 16-byte loads of W consecutive sites when the rows stay aligned
