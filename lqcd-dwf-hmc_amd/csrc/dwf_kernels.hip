@@ -831,7 +831,11 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             for (int r = 0; r < 4; ++r) {
               ya[r] = rounded<T>(ya[r] - ua * o1[r]);
               yb[r] = rounded<T>(yb[r] - ua * o2[r]);
-              nacc += (double)ya[r] * (double)ya[r] + (double)yb[r] * (double)yb[r];
+            }
+            if (nrm) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                nacc += (double)ya[r] * (double)ya[r] + (double)yb[r] * (double)yb[r];
             }
             Q4<T>::stn(upd + ob, ya);
             Q4<T>::stn(upd + ob + 12 * Ls, yb);
@@ -840,9 +844,11 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
               Q4<T>::stn(out + ob, o1);
               Q4<T>::stn(out + ob + 12 * Ls, o2);
             }
+            if (nrm) {   // (uniform: the norm is wanted by few launches)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              nacc += (double)o1[r] * (double)o1[r] + (double)o2[r] * (double)o2[r];
+              for (int r = 0; r < 4; ++r)
+                nacc += (double)o1[r] * (double)o1[r] + (double)o2[r] * (double)o2[r];
+            }
           }
         }
         if (OUT2) {
