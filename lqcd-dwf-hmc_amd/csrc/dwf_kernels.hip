@@ -450,7 +450,12 @@ __device__ __forceinline__ typename St<T>::C rounded(typename St<T>::C v) {
 #define DWF_WAVES_F32 4
 #endif
 #ifndef DWF_WAVES_F16
-#define DWF_WAVES_F16 3
+#define DWF_WAVES_F16 4
+#endif
+// FP16 storage: the chiral epilogue on the FP16 matrix cores (operands rounded
+// to FP16, FP32 accumulation) instead of the f32 16x16x4 chain
+#ifndef DWF_EPI_MFMA16
+#define DWF_EPI_MFMA16 1
 #endif
 // FP16 storage: the packed hop (dot2 SU(3) products) instead of the FP32 one
 #ifndef DWF_HOP_PK16
@@ -492,11 +497,17 @@ template <> struct KCfg<double> {
   static constexpr int WAVES = DWF_WAVES_F64;
 };
 
+// Occupancy target of one launch: 16-bit site-local launches keep 3 waves/SIMD
+// (their two-matrix epilogue needs the registers), hops use the storage type's.
+template <typename T, int OP> constexpr int k_waves() {
+  return (OP == K_SITE && sizeof(T) == 2) ? 3 : KCfg<T>::WAVES;
+}
+
 // LSC > 0: Ls and the block shape (THREADS / (LSC / VEC) sites) are compile-time
 // constants, so every load and LDS access uses an immediate offset; LSC = 0:
 // runtime Ls.  Thread (lsite, q) owns s = q*VEC .. q*VEC + VEC - 1 of its site.
 template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC, int VEC>
-__global__ void __launch_bounds__(KCfg<T>::THREADS, KCfg<T>::WAVES)
+__global__ void __launch_bounds__(KCfg<T>::THREADS, (k_waves<T, OP>()))
 k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links,
       const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
       const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
@@ -728,18 +739,40 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     v4 cu[NTW], cl[NTW];
     // D[j] (+)= M_chi B on the staged chirality
     auto mma_one = [&](const C* M, v4 (&D)[NTW], bool add) {
+      if constexpr (std::is_same<T, __half>::value && DWF_EPI_MFMA16) {
+        // FP16 storage: one v_mfma_f32_16x16x16f16 per tile, operands rounded to
+        // FP16 (lane l holds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]);
+        // the output layout is that of the f32 16x16x4 instruction
+        typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
+        h4_t av;
 #pragma unroll
-      for (int j = 0; j < NTW; ++j) {
-        const int t = wave + j * nwaves;
-        if (t < ntiles) {
-          const int col = t * 16 + li;
-          v4 a = add ? D[j] : v4{0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i) av[i] = (_Float16)M[(4 * lk + i) * 16 + arow];
 #pragma unroll
-          for (int kc = 0; kc < 4; ++kc) {
-            const int sin = kc * 4 + lk;
-            a = MM::run(M[sin * 16 + arow], mb[col * LP + sin], a);
+        for (int j = 0; j < NTW; ++j) {
+          const int t = wave + j * nwaves;
+          if (t < ntiles) {
+            const int col = t * 16 + li;
+            h4_t bv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[i] = (_Float16)mb[col * LP + 4 * lk + i];
+            D[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, add ? D[j] : v4{0, 0, 0, 0},
+                                                         0, 0, 0);
           }
-          D[j] = a;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+          const int t = wave + j * nwaves;
+          if (t < ntiles) {
+            const int col = t * 16 + li;
+            v4 a = add ? D[j] : v4{0, 0, 0, 0};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+              const int sin = kc * 4 + lk;
+              a = MM::run(M[sin * 16 + arow], mb[col * LP + sin], a);
+            }
+            D[j] = a;
+          }
         }
       }
     };
