@@ -452,8 +452,8 @@ __device__ __forceinline__ typename St<T>::C rounded(typename St<T>::C v) {
 #ifndef DWF_WAVES_F16
 #define DWF_WAVES_F16 4
 #endif
-// FP16 storage: the chiral epilogue on the FP16 matrix cores (operands rounded
-// to FP16, FP32 accumulation) instead of the f32 16x16x4 chain
+// 16-bit storage: the chiral epilogue on the FP16/BF16 matrix cores (operands
+// rounded to the storage type, FP32 accumulation) instead of the f32 16x16x4 chain
 #ifndef DWF_EPI_MFMA16
 #define DWF_EPI_MFMA16 1
 #endif
@@ -757,6 +757,29 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             for (int i = 0; i < 4; ++i) bv[i] = (_Float16)mb[col * LP + 4 * lk + i];
             D[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, add ? D[j] : v4{0, 0, 0, 0},
                                                          0, 0, 0);
+          }
+        }
+      } else if constexpr (std::is_same<T, __hip_bfloat16>::value && DWF_EPI_MFMA16) {
+        // BF16 storage: the same with v_mfma_f32_16x16x16_bf16 (operands rounded to
+        // BF16, round to nearest even)
+        typedef short s4_t __attribute__((ext_vector_type(4)));
+        auto bf = [](float f) -> short {
+          const __hip_bfloat16 h = __float2bfloat16(f);
+          return *reinterpret_cast<const short*>(&h);
+        };
+        s4_t av;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) av[i] = bf(M[(4 * lk + i) * 16 + arow]);
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+          const int t = wave + j * nwaves;
+          if (t < ntiles) {
+            const int col = t * 16 + li;
+            s4_t bv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[i] = bf(mb[col * LP + 4 * lk + i]);
+            D[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv,
+                                                             add ? D[j] : v4{0, 0, 0, 0}, 0, 0, 0);
           }
         }
       } else {
