@@ -738,65 +738,51 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     const int arow = MM::row(li);   // output s of this lane's A row
     v4 cu[NTW], cl[NTW];
     // D[j] (+)= M_chi B on the staged chirality
-    auto mma_one = [&](const C* M, v4 (&D)[NTW], bool add) {
+    // one 16-column tile: init + M_chi B, B = the tile's columns of the staged
+    // chirality (the columns of a tile are read only by the wave that owns it)
+    auto mma_tile = [&](const C* M, int col, v4 init) -> v4 {
       if constexpr (std::is_same<T, __half>::value && DWF_EPI_MFMA16) {
-        // FP16 storage: one v_mfma_f32_16x16x16f16 per tile, operands rounded to
-        // FP16 (lane l holds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]);
+        // FP16 storage: one v_mfma_f32_16x16x16f16, operands rounded to FP16
+        // (lane l holds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]);
         // the output layout is that of the f32 16x16x4 instruction
         typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
-        h4_t av;
+        h4_t av, bv;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = (_Float16)M[(4 * lk + i) * 16 + arow];
-#pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-          const int t = wave + j * nwaves;
-          if (t < ntiles) {
-            const int col = t * 16 + li;
-            h4_t bv;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) bv[i] = (_Float16)mb[col * LP + 4 * lk + i];
-            D[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, add ? D[j] : v4{0, 0, 0, 0},
-                                                         0, 0, 0);
-          }
+        for (int i = 0; i < 4; ++i) {
+          av[i] = (_Float16)M[(4 * lk + i) * 16 + arow];
+          bv[i] = (_Float16)mb[col * LP + 4 * lk + i];
         }
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, init, 0, 0, 0);
       } else if constexpr (std::is_same<T, __hip_bfloat16>::value && DWF_EPI_MFMA16) {
-        // BF16 storage: the same with v_mfma_f32_16x16x16_bf16 (operands rounded to
-        // BF16, round to nearest even)
+        // BF16 storage: the same with v_mfma_f32_16x16x16_bf16 (round to nearest even)
         typedef short s4_t __attribute__((ext_vector_type(4)));
         auto bf = [](float f) -> short {
           const __hip_bfloat16 h = __float2bfloat16(f);
           return *reinterpret_cast<const short*>(&h);
         };
-        s4_t av;
+        s4_t av, bv;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = bf(M[(4 * lk + i) * 16 + arow]);
-#pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-          const int t = wave + j * nwaves;
-          if (t < ntiles) {
-            const int col = t * 16 + li;
-            s4_t bv;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) bv[i] = bf(mb[col * LP + 4 * lk + i]);
-            D[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv,
-                                                             add ? D[j] : v4{0, 0, 0, 0}, 0, 0, 0);
-          }
+        for (int i = 0; i < 4; ++i) {
+          av[i] = bf(M[(4 * lk + i) * 16 + arow]);
+          bv[i] = bf(mb[col * LP + 4 * lk + i]);
         }
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, init, 0, 0, 0);
       } else {
+        v4 a = init;
 #pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-          const int t = wave + j * nwaves;
-          if (t < ntiles) {
-            const int col = t * 16 + li;
-            v4 a = add ? D[j] : v4{0, 0, 0, 0};
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) {
-              const int sin = kc * 4 + lk;
-              a = MM::run(M[sin * 16 + arow], mb[col * LP + sin], a);
-            }
-            D[j] = a;
-          }
+        for (int kc = 0; kc < 4; ++kc) {
+          const int sin = kc * 4 + lk;
+          a = MM::run(M[sin * 16 + arow], mb[col * LP + sin], a);
         }
+        return a;
+      }
+    };
+    // D[j] (+)= M_chi B on the staged chirality, every tile of this wave
+    auto mma_one = [&](const C* M, v4 (&D)[NTW], bool add) {
+#pragma unroll
+      for (int j = 0; j < NTW; ++j) {
+        const int t = wave + j * nwaves;
+        if (t < ntiles) D[j] = mma_tile(M, t * 16 + li, add ? D[j] : v4{0, 0, 0, 0});
       }
     };
     // core: u = acc_k + acc_{k+12} (P+), then l = acc_k - acc_{k+12} (P-)
@@ -839,8 +825,9 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       mma_one(sh_G + 256, cl, true);
     }
     // combine, add X, store; lane: column col, rows s_out = 4 lk + r.  With
-    // OUT2 the result is kept in cu/cl (as o_k + o_{k+12}, o_k - o_{k+12}) and
-    // staged again for F.
+    // OUT2 the tile's chiral combinations are staged in its own LDS columns and
+    // F is applied right away (after the last E product every wave reads only
+    // the columns of its own tiles, so no block barrier is needed).
     double nacc = 0;
 #pragma unroll
     for (int j = 0; j < NTW; ++j) {
@@ -907,60 +894,19 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             }
           }
         }
-        if (OUT2) {
+        if (OUT2) {   // out2 = F out, tile by tile in the wave's own LDS columns
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            cu[j][r] = o1[r] + o2[r];
-            cl[j][r] = o1[r] - o2[r];
-          }
-        }
-      }
-    }
-    if (nrm) {   // |out|^2: one partial per block (uniform branch)
-      __shared__ double wsum2[8];
-      const double t = wave_sum(nacc);
-      if (lane == 0) wsum2[wave] = t;
-      __syncthreads();
-      if (tid == 0) {
-        double a = 0;
-        for (int w = 0; w < nwaves; ++w) a += wsum2[w];
-        nrm[blockIdx.x] = a;
-      }
-    }
-    if (OUT2) {
-      v4 fu[NTW], fl[NTW];
-      auto stage_o = [&](const v4 (&src)[NTW]) {
+          for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] + o2[r];
+          const v4 fu = mma_tile(sh_F, col, v4{0, 0, 0, 0});
 #pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-          const int t = wave + j * nwaves;
-          if (t < ntiles) {
-            const int col = t * 16 + li;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = src[j][r];
-          }
-        }
-      };
-      __syncthreads();
-      stage_o(cu);
-      __syncthreads();
-      mma_one(sh_F, fu, false);
-      __syncthreads();
-      stage_o(cl);
-      __syncthreads();
-      mma_one(sh_F + 256, fl, false);
-#pragma unroll
-      for (int j = 0; j < NTW; ++j) {
-        const int t = wave + j * nwaves;
-        if (t < ntiles) {
-          const int col = t * 16 + li;
-          const int cs = col / 12, ck = col - cs * 12;
-          if (site0 + cs < Vh) {
-            const size_t ob = ((size_t)(site0 + cs) * 24 + ck) * Ls + 4 * lk;
+          for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] - o2[r];
+          const v4 fl = mma_tile(sh_F + 256, col, v4{0, 0, 0, 0});
+          if (cvalid) {
             C p1[4], p2[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              p1[r] = fu[j][r] + fl[j][r];
-              p2[r] = fu[j][r] - fl[j][r];
+              p1[r] = fu[r] + fl[r];
+              p2[r] = fu[r] - fl[r];
             }
             if (X2) {
               C xa[4], xb[4];
@@ -976,6 +922,17 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
             Q4<T>::stn(out2 + ob + 12 * Ls, p2);
           }
         }
+      }
+    }
+    if (nrm) {   // |out|^2: one partial per block (uniform branch)
+      __shared__ double wsum2[8];
+      const double t = wave_sum(nacc);
+      if (lane == 0) wsum2[wave] = t;
+      __syncthreads();
+      if (tid == 0) {
+        double a = 0;
+        for (int w = 0; w < nwaves; ++w) a += wsum2[w];
+        nrm[blockIdx.x] = a;
       }
     }
     return;
