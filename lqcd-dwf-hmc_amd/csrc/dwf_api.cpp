@@ -532,7 +532,7 @@ static int op_DdagD_cg(dwf_fopr_s* op, const void* p, const void* mpw, double* p
   e.nrm_partial = op->hop_partial;
   e.nrm_result = pap;
   unsigned nb2 = 0;
-  e.nrm_defer = true;   // added up by the next hop (256-thread blocks; the launch checks)
+  e.nrm_defer = true;   // added up by the last block of the next hop
   e.nblocks_out = &nb2;
   HIPCHK(launch_dwf(op->prec, K_HOP, nullptr, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
                     g_ctx.stream));

@@ -32,17 +32,18 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum of n partials by one 256-thread block, in the order k_red_finish uses
-// (thread t adds t, t + 256, ...; then the four wave sums in wave order), so
-// that every block that calls it, and k_red_finish, get the same value.
-__device__ __forceinline__ double block_sum256(const double* __restrict__ p, unsigned n,
-                                               double* sh4) {
+// Sum of n partials by one block of up to 8 waves, in a fixed order (thread t
+// adds t, t + blockDim, ...; then the wave sums in wave order); with 256
+// threads it is the order of k_red_finish.  sh holds 8 doubles.
+__device__ __forceinline__ double block_sum(const double* __restrict__ p, unsigned n,
+                                            double* sh) {
   double w = 0;
-  for (unsigned i = threadIdx.x; i < n; i += 256) w += p[i];
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) w += p[i];
   const double t = wave_sum(w);
-  if ((threadIdx.x & 63) == 0) sh4[threadIdx.x >> 6] = t;
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
   __syncthreads();
-  const double r = sh4[0] + sh4[1] + sh4[2] + sh4[3];
+  double r = 0;
+  for (unsigned k = 0; k < (blockDim.x >> 6); ++k) r += sh[k];
   __syncthreads();
   return r;
 }
@@ -481,7 +482,7 @@ template <typename T> struct KCfg {   // 16-bit storage
   static constexpr int WAVES = DWF_WAVES_F16;
 };
 #ifndef DWF_HOP_THREADS32
-#define DWF_HOP_THREADS32 DWF_HOP_THREADS
+#define DWF_HOP_THREADS32 512
 #endif
 template <> struct KCfg<float> {
   static constexpr int VEC = DWF_VEC_F32;
@@ -552,11 +553,10 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   const bool xmat = ADDX && MAT && G != nullptr;
 
   const int tid = threadIdx.x;
-  // a reduction left by the previous launch, finished by this launch's last
-  // block (256-thread blocks only: the host checks)
+  // a reduction left by the previous launch, finished by this launch's last block
   if (fin_partial && blockIdx.x == gridDim.x - 1) {
-    __shared__ double fsh[4];
-    const double t = block_sum256(fin_partial, fin_n, fsh);
+    __shared__ double fsh[8];
+    const double t = block_sum(fin_partial, fin_n, fsh);
     if (tid == 0) fin_result[0] = t;
   }
   const int lsite = tid / Q;
@@ -1219,10 +1219,10 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const bool mfma_path = mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap;
   if (e.upd && !(mfma_path && (op == K_SITE ? addx : !out2))) return hipErrorInvalidValue;
   if (!out && !e.upd && !(mfma_path && out2)) return hipErrorInvalidValue;
-  if (e.fin_partial && KCfg<T>::THREADS != 256) return hipErrorInvalidValue;
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
     const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
+    if (e.fin_partial && block.x % 64) return hipErrorInvalidValue;   // block_sum: whole waves
     *nblocks = grid.x;
     switch (Ls) {
       case 4: return launch_combo<T, 4>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr, parity, Vh, Ls, spb, e);
@@ -1237,6 +1237,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   if (lds_for(spb) > lds_cap) return hipErrorInvalidValue;
   const size_t lds = lds_for(spb);
   const dim3 block(spb * Ls), grid((Vh + spb - 1) / spb);
+  if (e.fin_partial && block.x % 64) return hipErrorInvalidValue;
   *nblocks = grid.x;
   return launch_combo<T, 0>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr,
                             parity, Vh, Ls, spb, e);
