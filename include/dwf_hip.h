@@ -124,6 +124,9 @@ int dwf_fopr_reverse_device(dwf_fopr op, double* v_corelib_device, const void* w
 int dwf_field_alloc(dwf_fopr op, void** field);
 int dwf_field_free(void* field);
 int dwf_field_set_zero(dwf_fopr op, void* x);
+/* AField::set(a): every stored real of x equals a
+ * (src/lib_alt_Accel/Field/afield.h:158, afield-tmpl.h:182-193). */
+int dwf_field_set(dwf_fopr op, void* x, double a);
 int dwf_field_copy(dwf_fopr op, void* y, const void* x);
 int dwf_field_axpy(dwf_fopr op, void* y, double a, const void* x);   /* y += a x   */
 int dwf_field_aypx(dwf_fopr op, double a, void* y, const void* x);   /* y = a y + x */
@@ -166,6 +169,28 @@ int dwf_solver_destroy(dwf_solver s);
 int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond,
                              int nshift, const double* sigma, void** x,
                              const void* b, int* nconv, double* diff);
+
+/* Batched multishift CG: nsys right-hand sides ("systems") solved together
+ * for the same shifts, each with the recurrences of AShiftsolver_CG::solve
+ * (ashiftsolver_CG-tmpl.h:89-286) -- e.g. the three shift solves of one
+ * AForce_F_Rational_Ratio::force_udiv (aforce_F_Rational_Ratio-tmpl.h:371,396,
+ * 420) or several pseudofermions, and independent lattices.  ops[k] is the
+ * operator of system k (mode DdagD; FP64 or FP32; all with the same precision,
+ * Ls and b, c; at most 4 different operators, which then need Ls = 12 or 16).
+ * Systems on one operator share every hop launch (links read once for all of
+ * them); different operators share grouped launches.  Scalars stay on the
+ * device; DWF_MSBATCH_GRAPH replays the iteration from a captured HIP graph.
+ * x: nsys * nshift fields of the system operators, x[k * nshift + j] solving
+ * (A_k + sigma_j) x = b[k]; nconv[k] as the reference reports it (-1 if not
+ * converged), diff[k] = max_j |(A_k + sigma_j) x_kj - b_k| / |b_k|. */
+typedef struct dwf_msbatch_s* dwf_msbatch;
+enum { DWF_MSBATCH_GRAPH = 1 };
+int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf_msbatch* out);
+int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* const* b,
+                      int max_iter, double stop_cond, int* nconv, double* diff);
+/* iterations of the last solve (the largest over the systems) */
+int dwf_msbatch_iterations(dwf_msbatch m);
+int dwf_msbatch_destroy(dwf_msbatch m);
 
 /* ------------------------------------------------------------------------
  * Learned polynomial preconditioner (no counterpart in the reference; it is a

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -871,6 +872,12 @@ int dwf_field_free(void* f) {
 int dwf_field_set_zero(dwf_fopr op, void* x) {
   CTX();
   HIPCHK(launch_set_zero(op->prec, x, op->nelem, g_ctx.stream));
+  return DWF_OK;
+}
+int dwf_field_set(dwf_fopr op, void* x, double a) {
+  CTX();
+  if (!op || !x) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_set_value(op->prec, x, a, op->nelem, g_ctx.stream));
   return DWF_OK;
 }
 int dwf_field_copy(dwf_fopr op, void* y, const void* x) {
@@ -1937,6 +1944,392 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
   release();
   return rc;
 }
+
+// ---- batched multishift CG ---------------------------------------------------
+// nsys systems (right-hand sides), each solved for the same nshift shifts with
+// the recurrences of AShiftsolver_CG (ashiftsolver_CG-tmpl.h:89-286), all
+// advanced together.  Systems on the same operator form one member: its nr
+// right-hand sides (padded to a power of two with zero slots) are contiguous
+// fields that every hop of D^+D processes in one launch, reading the links of
+// a site once for all of them.  Members on different operators (lattices of
+// different volume, same Ls and precision) run as one grouped launch per hop.
+// The scalars stay on the device (k_ms_ctrl); the host polls a status word in
+// coherent memory one iteration behind, or replays the iteration from a
+// captured HIP graph.
+struct MsMember {
+  dwf_fopr_s* op = nullptr;
+  int nrs = 0, npad = 1;          // 2^nrs fields per buffer, of which the first nr are systems
+  std::vector<int> sys;           // system index of each used slot
+  void* buf[7] = {};              // P0 S R T0 T1 U U2, each npad fields
+};
+enum { MB_P0, MB_S, MB_R, MB_T0, MB_T1, MB_U, MB_U2 };
+
+struct dwf_msbatch_s {
+  int nsys = 0, nshift = 0, prec = 0, Ls = 0, flags = 0;
+  bool mp_scalar = false;
+  std::vector<dwf_fopr_s*> ops;            // per system
+  std::vector<MsMember> mem;
+  std::vector<std::pair<int, int>> slot;   // system -> (member, slot)
+  std::vector<std::vector<void*>> p;       // shift search directions p_kj, j >= 1
+  // operator stages: per stage and member the launch, or one grouped launch
+  std::vector<std::vector<GroupMember>> stages;
+  std::vector<int> stage_op;
+  std::vector<GroupLaunch> glaunch;
+  // device tables and state
+  MsLaunch L;
+  void* dtab = nullptr;                    // all pointer tables and sizes
+  int* dcounter = nullptr;
+  double* hstatus = nullptr;               // coherent host memory
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int iters_total = 0;
+};
+
+// the D^+D chain of op_D (with v2) + op_Ddag (with pre) on the member buffers
+static void ms_build_stages(dwf_msbatch_s* m) {
+  auto add = [&](int kop, auto fill) {
+    std::vector<GroupMember> st(m->mem.size());
+    for (size_t g = 0; g < m->mem.size(); ++g) {
+      MsMember& M = m->mem[g];
+      GroupMember& G = st[g];
+      G.links = M.op->links;
+      G.nbr = M.op->nbr;
+      G.Vh = M.op->Vh;
+      G.e.nrs = M.nrs;
+      G.e.rstride = M.op->nelem;
+      fill(M, G);
+    }
+    m->stages.push_back(st);
+    m->stage_op.push_back(kop);
+  };
+  if (!m->mp_scalar)
+    add(K_SITE, [](MsMember& M, GroupMember& G) {
+      G.out = M.buf[MB_T0];
+      G.in = M.buf[MB_P0];
+      G.e.E = mat(M.op, M_PRE);
+    });
+  const bool pre = !m->mp_scalar;
+  add(K_HOP, [pre](MsMember& M, GroupMember& G) {
+    G.parity = 1;
+    G.out = M.buf[MB_T1];
+    G.in = pre ? M.buf[MB_T0] : M.buf[MB_P0];
+    G.e.E = mat(M.op, M_E1);
+  });
+  add(K_HOP, [](MsMember& M, GroupMember& G) {
+    G.parity = 0;
+    G.out = M.buf[MB_U];
+    G.in = M.buf[MB_T1];
+    G.e.E = mat(M.op, M_E2);
+    G.e.X = M.buf[MB_P0];
+    G.e.xcoef = 1.0;
+    G.e.out2 = M.buf[MB_U2];
+    G.e.F = mat(M.op, M_DIT);
+  });
+  add(K_HOPDAG, [](MsMember& M, GroupMember& G) {
+    G.parity = 1;
+    G.out = M.buf[MB_T1];
+    G.in = M.buf[MB_U2];
+    G.e.E = mat(M.op, M_G1);
+  });
+  add(K_HOPDAG, [](MsMember& M, GroupMember& G) {
+    G.parity = 0;
+    G.out = M.buf[MB_S];
+    G.in = M.buf[MB_T1];
+    G.e.E = mat(M.op, M_G2);
+    G.e.X = M.buf[MB_U];
+    G.e.xcoef = 1.0;
+  });
+}
+
+static int ms_run_operator(dwf_msbatch_s* m) {
+  for (size_t i = 0; i < m->stages.size(); ++i) {
+    if (m->mem.size() == 1) {
+      const GroupMember& G = m->stages[i][0];
+      HIPCHK(launch_dwf(m->prec, m->stage_op[i], G.out, G.in, G.links, G.nbr, G.parity, G.Vh,
+                        m->Ls, G.e, g_ctx.stream));
+    } else {
+      HIPCHK(launch_dwf_group(m->glaunch[i], g_ctx.stream));
+    }
+  }
+  return DWF_OK;
+}
+
+static int ms_run_iteration(dwf_msbatch_s* m) {
+  RC(ms_run_operator(m));
+  HIPCHK(launch_ms_blas(m->prec, m->L, 0, g_ctx.stream));
+  HIPCHK(launch_ms_blas(m->prec, m->L, 1, g_ctx.stream));
+  HIPCHK(launch_ms_ctrl(m->L, g_ctx.stream));
+  HIPCHK(launch_ms_blas(m->prec, m->L, 2, g_ctx.stream));
+  return DWF_OK;
+}
+
+static void ms_release(dwf_msbatch_s* m) {
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  for (auto& gl : m->glaunch) free_dwf_group(&gl);
+  for (auto& M : m->mem)
+    for (auto& b : M.buf)
+      if (b) (void)hipFree(b);
+  for (auto& v : m->p)
+    for (auto& q : v)
+      if (q) (void)hipFree(q);
+  if (m->dtab) (void)hipFree(m->dtab);
+  if (m->dcounter) (void)hipFree(m->dcounter);
+  if (m->L.state) (void)hipFree(m->L.state);
+  if (m->L.partial) (void)hipFree(m->L.partial);
+  if (m->L.pp_res) (void)hipFree(m->L.pp_res);
+  if (m->hstatus) (void)hipHostFree(m->hstatus);
+}
+
+extern "C" {
+int dwf_msbatch_destroy(dwf_msbatch m);
+
+int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf_msbatch* out) {
+  CTX();
+  if (!out || !ops || nsys < 1 || nsys > 64 || nshift < 1 || nshift > kMsShifts)
+    return fail(DWF_ERR_INVALID_ARG, "batched multishift: 1 <= nsys <= 64, 1 <= nshift <= 32");
+  *out = nullptr;
+  auto* m = new dwf_msbatch_s;
+  m->nsys = nsys;
+  m->nshift = nshift;
+  m->flags = flags;
+  auto bad = [&](int code, const char* msg) {
+    ms_release(m);
+    delete m;
+    return fail(code, msg);
+  };
+  for (int k = 0; k < nsys; ++k) {
+    dwf_fopr_s* op = ops[k];
+    if (!op) return bad(DWF_ERR_INVALID_ARG, "null operator");
+    if (!op->have_config) return bad(DWF_ERR_NO_CONFIG, "set_config has not been called");
+    if (op->prec != P64 && op->prec != P32)
+      return bad(DWF_ERR_UNSUPPORTED, "batched multishift runs on FP64 or FP32 operators");
+    if (k == 0) {
+      m->prec = op->prec;
+      m->Ls = op->Ls;
+      m->mp_scalar = op->mp_scalar;
+    } else if (op->prec != m->prec || op->Ls != m->Ls || op->mp_scalar != m->mp_scalar) {
+      return bad(DWF_ERR_UNSUPPORTED,
+                 "batched multishift: all operators need the same precision, Ls and b, c");
+    }
+    m->ops.push_back(op);
+    int g = -1;
+    for (size_t i = 0; i < m->mem.size(); ++i)
+      if (m->mem[i].op == op) g = (int)i;
+    if (g < 0) {
+      m->mem.emplace_back();
+      m->mem.back().op = op;
+      g = (int)m->mem.size() - 1;
+    }
+    m->slot.push_back({g, (int)m->mem[g].sys.size()});
+    m->mem[g].sys.push_back(k);
+  }
+  if (m->mem.size() > (size_t)kGroupMax)
+    return bad(DWF_ERR_UNSUPPORTED, "batched multishift: at most 4 different operators");
+  if (m->mem.size() > 1 && m->Ls != 12 && m->Ls != 16)
+    return bad(DWF_ERR_UNSUPPORTED, "several lattices in one launch need Ls = 12 or 16");
+  // member buffers
+  for (auto& M : m->mem) {
+    while ((1 << M.nrs) < (int)M.sys.size()) ++M.nrs;
+    M.npad = 1 << M.nrs;
+    const size_t fb = M.op->nelem * prec_bytes(M.op->prec) * M.npad;
+    for (int b = 0; b < 7; ++b) {
+      if (b == MB_T0 && m->mp_scalar) continue;
+      if (hipMalloc(&M.buf[b], fb) != hipSuccess) return bad(DWF_ERR_HIP, "device allocation failed");
+      if (hipMemset(M.buf[b], 0, fb) != hipSuccess) return bad(DWF_ERR_HIP, "hipMemset failed");
+    }
+  }
+  // shift directions
+  m->p.assign(nsys, std::vector<void*>(nshift, nullptr));
+  for (int k = 0; k < nsys; ++k)
+    for (int j = 1; j < nshift; ++j)
+      if (hipMalloc(&m->p[k][j], m->ops[k]->nelem * prec_bytes(m->prec)) != hipSuccess)
+        return bad(DWF_ERR_HIP, "device allocation failed");
+  ms_build_stages(m);
+  if (m->mem.size() > 1) {
+    m->glaunch.resize(m->stages.size());
+    for (size_t i = 0; i < m->stages.size(); ++i)
+      if (prepare_dwf_group(m->prec, m->stage_op[i], m->Ls, (int)m->mem.size(),
+                            m->stages[i].data(), &m->glaunch[i]) != hipSuccess)
+        return bad(DWF_ERR_UNSUPPORTED, "grouped launch could not be prepared");
+  }
+  // device tables: s p0 x0 r [nsys], x p [nsys][32], n [nsys]
+  const size_t ptab = sizeof(void*) * (4 * nsys + 2 * nsys * kMsShifts);
+  if (hipMalloc(&m->dtab, ptab + sizeof(size_t) * nsys) != hipSuccess)
+    return bad(DWF_ERR_HIP, "device allocation failed");
+  size_t maxn = 0;
+  for (auto* op : m->ops) maxn = std::max(maxn, op->nelem);
+  m->L.nsys = nsys;
+  m->L.nblk = (int)std::min<size_t>(kRedBlocks, (maxn / 2 + 255) / 256);
+  if (m->L.nblk < 1) m->L.nblk = 1;
+  void** t = (void**)m->dtab;
+  m->L.s = t;
+  m->L.p0 = t + nsys;
+  m->L.x0 = t + 2 * nsys;
+  m->L.r = t + 3 * nsys;
+  m->L.x = t + 4 * nsys;
+  m->L.p = t + 4 * nsys + nsys * kMsShifts;
+  m->L.n = (const size_t*)((char*)m->dtab + ptab);
+  if (hipMalloc(&m->L.state, sizeof(MsState) * nsys) != hipSuccess ||
+      hipMalloc(&m->L.partial, sizeof(double) * nsys * kMsShifts * m->L.nblk) != hipSuccess ||
+      hipMalloc(&m->L.pp_res, sizeof(double) * nsys * kMsShifts) != hipSuccess ||
+      hipMalloc(&m->dcounter, sizeof(int)) != hipSuccess)
+    return bad(DWF_ERR_HIP, "device allocation failed");
+  if (hipMemset(m->L.partial, 0, sizeof(double) * nsys * kMsShifts * m->L.nblk) != hipSuccess ||
+      hipMemset(m->L.pp_res, 0, sizeof(double) * nsys * kMsShifts) != hipSuccess ||
+      hipMemset(m->dcounter, 0, sizeof(int)) != hipSuccess)
+    return bad(DWF_ERR_HIP, "hipMemset failed");
+  if (hipHostMalloc(&m->hstatus, sizeof(double) * 4, hipHostMallocCoherent) != hipSuccess)
+    return bad(DWF_ERR_HIP, "pinned allocation failed");
+  m->L.status = m->hstatus;
+  m->L.counter = m->dcounter;
+  // fixed part of the pointer tables (the member buffers)
+  std::vector<void*> h(4 * nsys + 2 * nsys * kMsShifts, nullptr);
+  std::vector<size_t> hn(nsys);
+  for (int k = 0; k < nsys; ++k) {
+    const MsMember& M = m->mem[m->slot[k].first];
+    const size_t off = (size_t)m->slot[k].second * M.op->nelem * prec_bytes(m->prec);
+    h[k] = (char*)M.buf[MB_S] + off;
+    h[nsys + k] = (char*)M.buf[MB_P0] + off;
+    h[3 * nsys + k] = (char*)M.buf[MB_R] + off;
+    for (int j = 1; j < nshift; ++j) h[4 * nsys + nsys * kMsShifts + k * kMsShifts + j] = m->p[k][j];
+    hn[k] = M.op->nelem;
+  }
+  if (hipMemcpy(m->dtab, h.data(), ptab, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy((char*)m->dtab + ptab, hn.data(), sizeof(size_t) * nsys,
+                hipMemcpyHostToDevice) != hipSuccess)
+    return bad(DWF_ERR_HIP, "table upload failed");
+  *out = m;
+  return DWF_OK;
+}
+
+int dwf_msbatch_destroy(dwf_msbatch m) {
+  if (!m) return DWF_OK;
+  ms_release(m);
+  delete m;
+  return DWF_OK;
+}
+
+int dwf_msbatch_iterations(dwf_msbatch m) { return m ? m->iters_total : -1; }
+
+int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* const* b,
+                      int max_iter, double stop_cond, int* nconv, double* diff) {
+  CTX();
+  if (!m || !sigma || !x || !b || !nconv || !diff || max_iter < 1)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  for (auto* op : m->ops)
+    if (op->mode != "DdagD")
+      return fail(DWF_ERR_UNSUPPORTED, "batched multishift solves DdagD (set_mode(\"DdagD\"))");
+  hipStream_t st = g_ctx.stream;
+  const int nsys = m->nsys, nshift = m->nshift;
+  const size_t eb = prec_bytes(m->prec);
+  // |b|^2 and the start: r = p0 = p_j = b, x_j = 0
+  std::vector<MsState> hs(nsys);
+  for (int k = 0; k < nsys; ++k) {
+    dwf_fopr_s* op = m->ops[k];
+    double bn;
+    HIPCHK(launch_norm2(m->prec, b[k], op->nelem, g_ctx.red, 0, st));
+    RC(fetch(1, &bn));
+    if (!(bn > 0)) return fail(DWF_ERR_INVALID_ARG, "zero source");
+    const MsMember& M = m->mem[m->slot[k].first];
+    const size_t off = (size_t)m->slot[k].second * op->nelem * eb;
+    HIPCHK(launch_copy(m->prec, (char*)M.buf[MB_R] + off, b[k], op->nelem, st));
+    HIPCHK(launch_copy(m->prec, (char*)M.buf[MB_P0] + off, b[k], op->nelem, st));
+    for (int j = 0; j < nshift; ++j) {
+      if (j > 0) HIPCHK(launch_copy(m->prec, m->p[k][j], b[k], op->nelem, st));
+      HIPCHK(launch_set_zero(m->prec, x[k * nshift + j], op->nelem, st));
+    }
+    MsState& S = hs[k];
+    std::memset(&S, 0, sizeof(S));
+    S.rr = bn;
+    S.alpha_p = 0.0;
+    S.beta_p = 1.0;
+    S.snorm = 1.0 / bn;
+    S.sigma0 = sigma[0];
+    S.stop = stop_cond;
+    S.active = 1;
+    S.nshift2 = nshift;
+    S.nconv = -1;
+    S.max_iter = max_iter;
+    for (int j = 0; j < kMsShifts; ++j) {
+      S.zeta1[j] = S.zeta2[j] = 1.0;
+      S.csh2[j] = j < nshift ? sigma[j] - sigma[0] : 0.0;
+    }
+  }
+  // solution pointers of this solve
+  std::vector<void*> hx(2 * nsys + nsys * kMsShifts, nullptr);   // x0 [nsys], x [nsys][32]
+  for (int k = 0; k < nsys; ++k) {
+    hx[k] = x[k * nshift];
+    for (int j = 1; j < nshift; ++j) hx[nsys + k * kMsShifts + j] = x[k * nshift + j];
+  }
+  HIPCHK(hipMemcpyAsync(m->L.x0, hx.data(), sizeof(void*) * nsys, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(m->L.x, hx.data() + nsys, sizeof(void*) * nsys * kMsShifts,
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(m->L.state, hs.data(), sizeof(MsState) * nsys, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int cnt0 = 0;
+  HIPCHK(hipMemcpy(&cnt0, m->dcounter, sizeof(int), hipMemcpyDeviceToHost));
+  m->hstatus[0] = (double)cnt0 - 1.0;
+  m->hstatus[1] = nsys;
+  const bool use_graph = (m->flags & DWF_MSBATCH_GRAPH) != 0;
+  if (use_graph && !m->gexec) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = ms_run_iteration(m);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != DWF_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(e);
+    m->graph = g;
+    HIPCHK(hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
+  }
+  // iterations: the host enqueues iteration i + 1 before it waits for the
+  // status of iteration i (kernels of inactive systems return at once)
+  volatile double* status = m->hstatus;
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    if (use_graph) HIPCHK(hipGraphLaunch(m->gexec, st));
+    else RC(ms_run_iteration(m));
+    if (it >= 1) {
+      const double want = (double)(cnt0 + it - 1);
+      for (long spin = 0; status[0] < want; ++spin)
+        if ((spin & 1023) == 1023 && hipStreamQuery(st) == hipSuccess) break;   // stream idle
+      if (status[1] < 0.5) break;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(hs.data(), m->L.state, sizeof(MsState) * nsys, hipMemcpyDeviceToHost));
+  m->iters_total = 0;
+  bool all = true;
+  for (int k = 0; k < nsys; ++k) {
+    nconv[k] = hs[k].conv ? hs[k].nconv : -1;
+    all = all && hs[k].conv;
+    m->iters_total = std::max(m->iters_total, hs[k].nconv + 1);
+  }
+  // diff (the reference's check after the iteration): max over the shifts of
+  // |(A + sigma_j) x_j - b| / |b|, with the operator itself
+  for (int k = 0; k < nsys; ++k) {
+    dwf_fopr_s* op = m->ops[k];
+    void* s;
+    RC(get_work(op, 4, &s));
+    double dmax = -1.0;
+    for (int j = 0; j < nshift; ++j) {
+      RC(mult_impl(op, s, x[k * nshift + j], op->mode, false));
+      HIPCHK(launch_axpy(op->prec, s, sigma[j], x[k * nshift + j], op->nelem, st));
+      double d2;
+      HIPCHK(launch_sub_norm2(op->prec, s, s, b[k], op->nelem, g_ctx.red, 0, st));
+      RC(fetch(1, &d2));
+      const double d1 = std::sqrt(d2 * hs[k].snorm);
+      if (d1 > dmax) dmax = d1;
+    }
+    diff[k] = dmax;
+  }
+  return all ? DWF_OK : fail(DWF_ERR_NOT_CONVERGED, "batched multishift: not converged");
+}
+}  // extern "C"
 
 // ---- DWF fermion force (FP64) -----------------------------------------------
 // AForce_F_Domainwall_5din_eo (aforce_F_Domainwall_5din_eo-tmpl.h:299-451) with

@@ -1,0 +1,312 @@
+// dwf_batch.hip -- kernels of the batched multishift CG.
+//
+// Several right-hand sides ("systems": pseudofermion sources, possibly on
+// different lattices), each with up to kMsShifts shifts, advance together: one
+// launch per stage covers every system (blockIdx.y = system), and the scalar
+// recurrences of AShiftsolver_CG::solve_step (ashiftsolver_CG-tmpl.h:216-286)
+// run on the device (k_ms_ctrl), so that an iteration needs no host
+// synchronisation and can be replayed from a captured graph.
+//
+// Per iteration, for every active system k:
+//   [operator]  s_k = A p0_k                       (multi-RHS / grouped hops)
+//   k_ms_pa     s_k += sigma0 p0_k; pa_k = <s_k, p0_k>
+//   k_ms_xr     x0_k += (rr_k / pa_k) p0_k; r_k -= (rr_k / pa_k) s_k; |r_k|^2
+//   k_ms_ctrl   truncation of the shifts by the previous |p_j|^2, beta, alpha,
+//               zeta_j and the update coefficients, convergence
+//   k_ms_shift  p0_k = r_k + alpha p0_k; for every active shift j >= 1:
+//               x_kj -= beta_j p_kj; p_kj = alpha_j p_kj + zeta_j r_k; |p_kj|^2
+// with r read once per element for all shifts.  Reductions are deterministic
+// (block partials, then k_ms_finish adds them in a fixed order).
+#include "dwf_device.h"
+#include "dwf_kernels.h"
+
+namespace dwf {
+
+// two consecutive reals of a field (FP64: one 16-byte access, FP32: 8 bytes)
+template <typename T> struct Vec2;
+template <> struct Vec2<double> {
+  static __device__ __forceinline__ void ld(const double* p, double* o) {
+    const dbl2_t v = *reinterpret_cast<const dbl2_t*>(p);
+    o[0] = v[0];
+    o[1] = v[1];
+  }
+  static __device__ __forceinline__ void st(double* p, const double* v) {
+    *reinterpret_cast<dbl2_t*>(p) = dbl2_t{v[0], v[1]};
+  }
+};
+template <> struct Vec2<float> {
+  static __device__ __forceinline__ void ld(const float* p, float* o) {
+    const flt2_t v = *reinterpret_cast<const flt2_t*>(p);
+    o[0] = v[0];
+    o[1] = v[1];
+  }
+  static __device__ __forceinline__ void st(float* p, const float* v) {
+    *reinterpret_cast<flt2_t*>(p) = flt2_t{v[0], v[1]};
+  }
+};
+
+__device__ __forceinline__ double ms_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// 256-thread block: the block's sum of v, valid in thread 0
+__device__ __forceinline__ double ms_block_sum(double v, double* sh) {
+  const double t = ms_wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+  __syncthreads();
+  return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+#define MS_LOOP(n)                                                                     \
+  for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 2; i < (n);        \
+       i += (size_t)gridDim.x * blockDim.x * 2)
+
+// s += sigma0 p0;  <s, p0>  (fields of an even number of reals)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_ms_pa(T* const* __restrict__ s, T* const* __restrict__ p0, const size_t* __restrict__ n,
+        const MsState* __restrict__ st, double* __restrict__ partial) {
+  using C = typename St<T>::C;
+  const int k = blockIdx.y;
+  if (!st[k].active) return;
+  const C sg = (C)st[k].sigma0;
+  T* sk = s[k];
+  const T* pk = p0[k];
+  double v = 0;
+  MS_LOOP(n[k]) {
+    C sv[2], pv[2];
+    Vec2<T>::ld(sk + i, sv);
+    Vec2<T>::ld(pk + i, pv);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      sv[j] = (sv[j] + sg * pv[j]);
+      v += (double)sv[j] * (double)pv[j];
+    }
+    Vec2<T>::st(sk + i, sv);
+  }
+  __shared__ double sh[4];
+  const double t = ms_block_sum(v, sh);
+  if (threadIdx.x == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = t;
+}
+
+// x0 += a p0;  r -= a s;  |r|^2,  a = rr / pa
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_ms_xr(T* const* __restrict__ x0, T* const* __restrict__ r, T* const* __restrict__ p0,
+        T* const* __restrict__ s, const size_t* __restrict__ n, const MsState* __restrict__ st,
+        double* __restrict__ partial) {
+  using C = typename St<T>::C;
+  const int k = blockIdx.y;
+  if (!st[k].active) return;
+  const C a = (C)(st[k].rr / st[k].pa);
+  T* xk = x0[k];
+  T* rk = r[k];
+  const T* pk = p0[k];
+  const T* sk = s[k];
+  double v = 0;
+  MS_LOOP(n[k]) {
+    C xv[2], rv[2], pv[2], sv[2];
+    Vec2<T>::ld(xk + i, xv);
+    Vec2<T>::ld(rk + i, rv);
+    Vec2<T>::ld(pk + i, pv);
+    Vec2<T>::ld(sk + i, sv);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      xv[j] += a * pv[j];
+      rv[j] = (rv[j] - a * sv[j]);
+      v += (double)rv[j] * (double)rv[j];
+    }
+    Vec2<T>::st(xk + i, xv);
+    Vec2<T>::st(rk + i, rv);
+  }
+  __shared__ double sh[4];
+  const double t = ms_block_sum(v, sh);
+  if (threadIdx.x == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = t;
+}
+
+// p0 = r + alpha p0; shifts j = 1 .. nshift2 - 1 of the system with the
+// coefficients of k_ms_ctrl; |p_j|^2 block partials at
+// partial[((k * kMsShifts) + j) * gridDim.x + block].
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_ms_shift(T* const* __restrict__ x, T* const* __restrict__ p, T* const* __restrict__ p0,
+           T* const* __restrict__ r, const size_t* __restrict__ n,
+           const MsState* __restrict__ st, double* __restrict__ partial) {
+  using C = typename St<T>::C;
+  const int k = blockIdx.y;
+  const MsState& S = st[k];
+  if (!S.upd) return;
+  const int m = S.nshift2;
+  const C alpha = (C)S.coef[1];
+  T* const* xk = x + (size_t)k * kMsShifts;
+  T* const* pk = p + (size_t)k * kMsShifts;
+  T* p0k = p0[k];
+  const T* rk = r[k];
+  double v[kMsShifts];
+#pragma unroll
+  for (int j = 0; j < kMsShifts; ++j) v[j] = 0;
+  MS_LOOP(n[k]) {
+    C rv[2], qv[2];
+    Vec2<T>::ld(rk + i, rv);
+    Vec2<T>::ld(p0k + i, qv);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) qv[e] = rv[e] + alpha * qv[e];
+    Vec2<T>::st(p0k + i, qv);
+#pragma unroll
+    for (int j = 1; j < kMsShifts; ++j) {
+      if (j < m) {
+        T* xj = xk[j];
+        T* pj = pk[j];
+        const C c0 = (C)S.coef[3 * j], c1 = (C)S.coef[3 * j + 1], c2 = (C)S.coef[3 * j + 2];
+        C xv[2], pv[2];
+        Vec2<T>::ld(xj + i, xv);
+        Vec2<T>::ld(pj + i, pv);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          xv[e] += c0 * pv[e];
+          pv[e] = (c1 * pv[e] + c2 * rv[e]);
+          v[j] += (double)pv[e] * (double)pv[e];
+        }
+        Vec2<T>::st(xj + i, xv);
+        Vec2<T>::st(pj + i, pv);
+      }
+    }
+  }
+  __shared__ double sh[4];
+#pragma unroll
+  for (int j = 1; j < kMsShifts; ++j) {
+    if (j < m) {
+      const double t = ms_block_sum(v[j], sh);
+      if (threadIdx.x == 0)
+        partial[((size_t)k * kMsShifts + j) * gridDim.x + blockIdx.x] = t;
+      __syncthreads();
+    }
+  }
+}
+
+// out[k * ostride + o] = sum of the nblk partials of (system k, output o),
+// in a fixed order; block (o, k)
+__global__ void __launch_bounds__(256)
+k_ms_finish(const double* __restrict__ partial, int nblk, int nout, double* out, int ostride) {
+  const int o = blockIdx.x, k = blockIdx.y;
+  const double* p = partial + ((size_t)k * nout + o) * nblk;
+  double w = 0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) w += p[i];
+  __shared__ double sh[4];
+  const double t = ms_block_sum(w, sh);
+  if (threadIdx.x == 0) out[(size_t)k * ostride + o] = t;
+}
+
+// The scalar part of solve_step for every system (one thread each), then the
+// status for the host: status[0] = iteration, status[1] = active systems
+// (coherent host memory, written after a system-scope fence).
+// pp_res: |p_j|^2 of the previous k_ms_shift (unscaled), [k][kMsShifts].
+__global__ void __launch_bounds__(64)
+k_ms_ctrl(MsState* __restrict__ st, const double* __restrict__ pp_res, int nsys, int* counter,
+          double* status) {
+  const int k = threadIdx.x;
+  __shared__ int nact;
+  if (k == 0) nact = 0;
+  __syncthreads();
+  if (k < nsys) {
+    MsState& S = st[k];
+    S.upd = 0;
+    if (S.active) {
+      if (S.nconv >= 0) {
+        // shift truncation after the previous update (from the top)
+        for (int j = 1; j < S.nshift2; ++j) S.pp[j] = pp_res[k * kMsShifts + j] * S.snorm;
+        int ish1 = S.nshift2;
+        for (int j = S.nshift2 - 1; j >= 0; --j)
+          if (S.pp[j] > S.stop) {
+            ish1 = j + 1;
+            break;
+          }
+        S.nshift2 = ish1;
+      }
+      const double rr_p = S.rr;
+      const double beta = -rr_p / S.pa;
+      const double rr = S.rr_new;
+      const double alpha = rr / rr_p;
+      S.pp[0] = rr;
+      const double alpha_h = 1.0 + S.alpha_p * beta / S.beta_p;
+      S.coef[0] = 0.0;
+      S.coef[1] = alpha;
+      S.coef[2] = 1.0;
+      for (int j = 1; j < S.nshift2; ++j) {
+        double zeta = (alpha_h - S.csh2[j] * beta) / S.zeta1[j] + (1.0 - alpha_h) / S.zeta2[j];
+        zeta = 1.0 / zeta;
+        const double zr = zeta / S.zeta1[j];
+        S.coef[3 * j] = -beta * zr;
+        S.coef[3 * j + 1] = alpha * zr * zr;
+        S.coef[3 * j + 2] = zeta;
+        S.zeta2[j] = S.zeta1[j];
+        S.zeta1[j] = zeta;
+      }
+      S.rr = rr;
+      S.alpha_p = alpha;
+      S.beta_p = beta;
+      S.nconv += 1;
+      S.upd = 1;
+      if (rr * S.snorm < S.stop) {
+        S.active = 0;
+        S.conv = 1;
+      } else if (S.nconv + 1 >= S.max_iter) {
+        S.active = 0;   // iteration limit reached, not converged
+      }
+    }
+    if (S.active) atomicAdd(&nact, 1);
+  }
+  __syncthreads();
+  if (k == 0) {
+    const int iter = counter[0];
+    counter[0] = iter + 1;
+    status[1] = (double)nact;
+    __threadfence_system();
+    status[0] = (double)iter;
+    __threadfence_system();
+  }
+}
+
+template <typename T>
+static hipError_t ms_iter_blas_t(const MsLaunch& L, int stage, hipStream_t st) {
+  const dim3 g(L.nblk, L.nsys), b(256);
+  switch (stage) {
+    case 0:
+      hipLaunchKernelGGL(k_ms_pa<T>, g, b, 0, st, (T* const*)L.s, (T* const*)L.p0, L.n, L.state,
+                         L.partial);
+      hipLaunchKernelGGL(k_ms_finish, dim3(1, L.nsys), b, 0, st, L.partial, L.nblk, 1,
+                         &L.state->pa, (int)(sizeof(MsState) / sizeof(double)));
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_ms_xr<T>, g, b, 0, st, (T* const*)L.x0, (T* const*)L.r,
+                         (T* const*)L.p0, (T* const*)L.s, L.n, L.state, L.partial);
+      hipLaunchKernelGGL(k_ms_finish, dim3(1, L.nsys), b, 0, st, L.partial, L.nblk, 1,
+                         &L.state->rr_new, (int)(sizeof(MsState) / sizeof(double)));
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_ms_shift<T>, g, b, 0, st, (T* const*)L.x, (T* const*)L.p,
+                         (T* const*)L.p0, (T* const*)L.r, L.n, L.state, L.partial);
+      hipLaunchKernelGGL(k_ms_finish, dim3(kMsShifts, L.nsys), b, 0, st, L.partial, L.nblk,
+                         kMsShifts, L.pp_res, kMsShifts);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ms_blas(int prec, const MsLaunch& L, int stage, hipStream_t st) {
+  if (prec == P64) return ms_iter_blas_t<double>(L, stage, st);
+  if (prec == P32) return ms_iter_blas_t<float>(L, stage, st);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ms_ctrl(const MsLaunch& L, hipStream_t st) {
+  if (L.nsys > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ms_ctrl, dim3(1), dim3(64), 0, st, L.state, L.pp_res, L.nsys, L.counter,
+                     L.status);
+  return hipGetLastError();
+}
+
+}  // namespace dwf

@@ -59,7 +59,70 @@ struct Epi {
   const double* fin_partial = nullptr;
   unsigned fin_n = 0;
   double* fin_result = nullptr;
+  // several right-hand sides of one lattice in one launch: 2^nrs fields, each
+  // rstride reals after the previous one, for in, out, X, out2, X2 and upd;
+  // their spinors of one 4D site share the staged links (Vh stays the 4D count)
+  int nrs = 0;
+  size_t rstride = 0;
 };
+
+// Grouped launch: up to kGroupMax lattices (same Ls, precision and epilogue
+// form; compile-time Ls 12 or 16; FP64 or FP32) in one launch of the fused
+// kernel.  The member table lives in device memory and is built once.
+constexpr int kGroupMax = 4;
+struct GroupMember {
+  void* out = nullptr;
+  const void* in = nullptr;
+  const void* links = nullptr;
+  const int* nbr = nullptr;
+  int parity = 0;
+  int Vh = 0;
+  Epi e;
+};
+struct GroupLaunch {
+  int prec = 0, op = 0, Ls = 0, n = 0;
+  bool mat = false, addx = false, out2 = false;
+  unsigned grid = 0, block = 0;
+  size_t lds = 0;
+  void* table = nullptr;   // device memory (owned)
+};
+// Batched multishift CG (dwf_batch.hip): device state of one system (one
+// right-hand side with its shifts), the scalars of AShiftsolver_CG
+// (ashiftsolver_CG-tmpl.h:216-286).
+constexpr int kMsShifts = 32;
+struct MsState {
+  double rr, rr_new, pa, alpha_p, beta_p, snorm, sigma0, stop;
+  int active, upd, conv, nshift2, nconv, max_iter;
+  double csh2[kMsShifts], zeta1[kMsShifts], zeta2[kMsShifts], pp[kMsShifts];
+  double coef[3 * kMsShifts];   // per shift: x += c0 p, p = c1 p + c2 r
+};
+// Device pointer tables of one batched solve (all in device memory):
+// s, p0, x0, r: [nsys]; x, p: [nsys][kMsShifts] (slot 0 unused); n: reals per
+// field of each system; partial: nsys * kMsShifts * nblk doubles; pp_res:
+// nsys * kMsShifts; status: coherent host memory (2 doubles).
+struct MsLaunch {
+  int nsys = 0, nblk = 0;
+  void** s = nullptr;
+  void** p0 = nullptr;
+  void** x0 = nullptr;
+  void** r = nullptr;
+  void** x = nullptr;
+  void** p = nullptr;
+  const size_t* n = nullptr;
+  MsState* state = nullptr;
+  double* partial = nullptr;
+  double* pp_res = nullptr;
+  double* status = nullptr;
+  int* counter = nullptr;   // iterations done (device), copied to status[0]
+};
+// stage 0: s += sigma0 p0, pa;  1: x0/r update, rr_new;  2: p0 and the shifts
+hipError_t launch_ms_blas(int prec, const MsLaunch& L, int stage, hipStream_t st);
+hipError_t launch_ms_ctrl(const MsLaunch& L, hipStream_t st);
+
+hipError_t prepare_dwf_group(int prec, int op, int Ls, int n, const GroupMember* m,
+                             GroupLaunch* gl);
+hipError_t launch_dwf_group(const GroupLaunch& gl, hipStream_t st);
+void free_dwf_group(GroupLaunch* gl);
 
 hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* links,
                       const int* nbr, int parity, int Vh, int Ls, const Epi& e,
@@ -76,6 +139,7 @@ hipError_t launch_pack_links(int prec, void* links, const double* U, int Nx, int
 // BLAS-1 on n elements of the storage type (arithmetic in the compute type,
 // reductions in double).
 hipError_t launch_set_zero(int prec, void* x, size_t n, hipStream_t st);
+hipError_t launch_set_value(int prec, void* x, double a, size_t n, hipStream_t st);
 hipError_t launch_copy(int prec, void* y, const void* x, size_t n, hipStream_t st);
 hipError_t launch_axpy(int prec, void* y, double a, const void* x, size_t n, hipStream_t st);
 hipError_t launch_aypx(int prec, double a, void* y, const void* x, size_t n, hipStream_t st);

@@ -15,6 +15,7 @@
 // the block.  The Ls threads of a site read contiguous spinor memory and share
 // (by broadcast) the site's links.
 #include <type_traits>
+#include <vector>
 
 #include "dwf_device.h"
 #include "dwf_kernels.h"
@@ -25,6 +26,14 @@
 #endif
 
 namespace dwf {
+
+// Orders this wave's LDS stores before its later LDS loads in every lane
+// (data handed between lanes of one wave through LDS).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -507,16 +516,24 @@ template <typename T, int OP> constexpr int k_waves() {
 // LSC > 0: Ls and the block shape (THREADS / (LSC / VEC) sites) are compile-time
 // constants, so every load and LDS access uses an immediate offset; LSC = 0:
 // runtime Ls.  Thread (lsite, q) owns s = q*VEC .. q*VEC + VEC - 1 of its site.
+//
+// Several right-hand sides on one lattice (nr = 2^nrs of them, each a whole
+// field, rstride reals after the previous one) run as one launch: the kernel
+// walks "virtual sites" v = site * nr + rhs, so the nr spinors of a 4D site sit
+// in the same block and share its staged links (read once for all of them).
+// blk / nblk: this block's index among the launch's blocks (after the XCD
+// remapping) and their number.
 template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC, int VEC>
-__global__ void __launch_bounds__(KCfg<T>::THREADS, (k_waves<T, OP>()))
-k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links,
-      const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
-      const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
-      const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
-      const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
-      const T* X2, double* __restrict__ nrm, const double* xnum, const double* xden,
-      T* __restrict__ upd, const double* unum, const double* uden,
-      const double* fin_partial, unsigned fin_n, double* fin_result) {
+__device__ __forceinline__ void
+dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links,
+         const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
+         const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
+         const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
+         const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
+         const T* X2, double* __restrict__ nrm, const double* xnum, const double* xden,
+         T* __restrict__ upd, const double* unum, const double* uden,
+         const double* fin_partial, unsigned fin_n, double* fin_result, int blk, int nblk,
+         int nrs, size_t rstride) {
   using S = St<T>;
   using C = typename S::C;
   using VL = Vx<T, VEC>;
@@ -554,18 +571,28 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
 
   const int tid = threadIdx.x;
   // a reduction left by the previous launch, finished by this launch's last block
-  if (fin_partial && blockIdx.x == gridDim.x - 1) {
+  if (fin_partial && blk == nblk - 1) {
     __shared__ double fsh[8];
     const double t = block_sum(fin_partial, fin_n, fsh);
     if (tid == 0) fin_result[0] = t;
   }
   const int lsite = tid / Q;
   const int s0 = (tid - lsite * Q) * VEC;
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  // virtual sites (site * nr + rhs) of this block; the 4D sites they cover
+  // are staged: rsite0 .. rsite0 + nsite - 1, this thread's one at row rl
+  const int VhV = Vh << nrs;
+  const int rmask = (1 << nrs) - 1;
   const int site0 = blk * spb;
   const int site = site0 + lsite;
-  const bool valid = (lsite < spb) && (site < Vh);
-  const int nsite = (Vh - site0) < spb ? (Vh - site0) : spb;
+  const bool valid = (lsite < spb) && (site < VhV);
+  const int rsite0 = site0 >> nrs;
+  const int vend = (site0 + spb < VhV ? site0 + spb : VhV) - 1;
+  const int nsite = (vend >> nrs) - rsite0 + 1;
+  const int rl = (site >> nrs) - rsite0;
+  // element offset of virtual site v (rhs field, 4D site)
+  auto vaddr = [&](int v) -> size_t {
+    return (size_t)(v & rmask) * rstride + (size_t)(v >> nrs) * 24 * Ls;
+  };
 
   if (MAT) {
     const int LL = Ls * Ls;
@@ -579,13 +606,13 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   }
   if (OP != K_SITE) {
     // coalesced cooperative load of the block's 8 x 18 link rows and 8 neighbour rows
-    const T* L = links + (size_t)parity * 8 * 18 * Vh + site0;
+    const T* L = links + (size_t)parity * 8 * 18 * Vh + rsite0;
     if constexpr (PK) {
       unsigned* shP = reinterpret_cast<unsigned*>(shA);
       const unsigned short* Lh = reinterpret_cast<const unsigned short*>(L);
       // eight sites per 16-byte load when the rows stay 16-byte aligned (Vh and
       // spb multiples of 8), else one site at a time
-      if ((Vh & 7) == 0 && (spb & 7) == 0) {
+      if ((Vh & 7) == 0 && (spb & 7) == 0 && (rsite0 & 7) == 0) {
         const int h8 = spb >> 3;
         for (int i = tid; i < 72 * h8; i += blockDim.x) {
           const int row = i / h8, j = (i - row * h8) * 8;   // row = dir * 9 + entry
@@ -629,7 +656,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       // 16-byte loads of W consecutive sites when the rows stay aligned
       constexpr int W = 16 / sizeof(T);
       typedef T tw_t __attribute__((ext_vector_type(W)));
-      if ((Vh % W) == 0 && (spb % W) == 0) {
+      if ((Vh % W) == 0 && (spb % W) == 0 && (rsite0 % W) == 0) {
         const int sw = spb / W;
         for (int i = tid; i < 144 * sw; i += blockDim.x) {
           const int row = i / sw, j = (i - row * sw) * W;
@@ -656,7 +683,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
         shL[i] = (j < nsite) ? S::ldn(L + (size_t)row * Vh + j) : C(0);
       }
     }
-    const int* N = nbr + (size_t)parity * 8 * Vh + site0;
+    const int* N = nbr + (size_t)parity * 8 * Vh + rsite0;
     for (int i = tid; i < 8 * spb; i += blockDim.x) {
       const int row = i / spb, j = i - row * spb;
       shN[i] = (j < nsite) ? N[(size_t)row * Vh + j] : 0;
@@ -669,7 +696,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   for (int v = 0; v < VEC; ++v)
 #pragma unroll
     for (int k = 0; k < 24; ++k) acc[v][k] = 0;
-  const size_t base = (size_t)site * 24 * Ls + s0;
+  const size_t base = vaddr(site) + s0;
+  const T* inr = in + (size_t)(site & rmask) * rstride;   // this rhs's input field
   if (valid) {
     if (OP == K_SITE) {
 #pragma unroll
@@ -680,16 +708,16 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       }
     } else if (OP == K_HOP) {
       if constexpr (PK)
-        hop_all_pk<false>(acc, reinterpret_cast<const __half*>(in),
-                          reinterpret_cast<const unsigned*>(shA), shN, lsite, s0, spb);
+        hop_all_pk<false>(acc, reinterpret_cast<const __half*>(inr),
+                          reinterpret_cast<const unsigned*>(shA), shN, rl, s0, spb);
       else
-        hop_all<false, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+        hop_all<false, T, VEC>(acc, inr, shL, shN, rl, s0, spb, Ls);
     } else {
       if constexpr (PK)
-        hop_all_pk<true>(acc, reinterpret_cast<const __half*>(in),
-                         reinterpret_cast<const unsigned*>(shA), shN, lsite, s0, spb);
+        hop_all_pk<true>(acc, reinterpret_cast<const __half*>(inr),
+                         reinterpret_cast<const unsigned*>(shA), shN, rl, s0, spb);
       else
-        hop_all<true, T, VEC>(acc, in, shL, shN, lsite, s0, spb, Ls);
+        hop_all<true, T, VEC>(acc, inr, shL, shN, rl, s0, spb, Ls);
 #pragma unroll
       for (int v = 0; v < VEC; ++v)
 #pragma unroll
@@ -835,8 +863,8 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       if (t < ntiles) {
         const int col = t * 16 + li;
         const int cs = col / 12, ck = col - cs * 12;
-        const bool cvalid = site0 + cs < Vh;
-        const size_t ob = ((size_t)(site0 + cs) * 24 + ck) * Ls + 4 * lk;
+        const bool cvalid = site0 + cs < VhV;
+        const size_t ob = vaddr(site0 + cs) + (size_t)ck * Ls + 4 * lk;
         C o1[4], o2[4];   // rows k and k + 12
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -895,11 +923,17 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
           }
         }
         if (OUT2) {   // out2 = F out, tile by tile in the wave's own LDS columns
+          // The FP32/FP64 chain reads rows written by other lanes of the wave,
+          // and the second staging overwrites rows other lanes have just read:
+          // each hand-over is a wave-scope release/acquire around a wave barrier.
 #pragma unroll
           for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] + o2[r];
+          wave_lds_sync();
           const v4 fu = mma_tile(sh_F, col, v4{0, 0, 0, 0});
+          wave_lds_sync();
 #pragma unroll
           for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] - o2[r];
+          wave_lds_sync();
           const v4 fl = mma_tile(sh_F + 256, col, v4{0, 0, 0, 0});
           if (cvalid) {
             C p1[4], p2[4];
@@ -932,7 +966,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
       if (tid == 0) {
         double a = 0;
         for (int w = 0; w < nwaves; ++w) a += wsum2[w];
-        nrm[blockIdx.x] = a;
+        nrm[blk] = a;
       }
     }
     return;
@@ -1059,7 +1093,7 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
     if (tid == 0) {
       double a = 0;
       for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) a += wsum[w];
-      nrm[blockIdx.x] = a;
+      nrm[blk] = a;
     }
   }
   if (OUT2) {
@@ -1130,6 +1164,55 @@ k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links
   }
 }
 
+template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC, int VEC>
+__global__ void __launch_bounds__(KCfg<T>::THREADS, (k_waves<T, OP>()))
+k_dwf(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ links,
+      const int* __restrict__ nbr, int parity, int Vh, int Ls_rt, int spb_rt,
+      const typename St<T>::C* __restrict__ E, typename St<T>::C scale,
+      const T* __restrict__ X, typename St<T>::C xcoef, T* __restrict__ out2,
+      const typename St<T>::C* __restrict__ F, const typename St<T>::C* __restrict__ G,
+      const T* X2, double* __restrict__ nrm, const double* xnum, const double* xden,
+      T* __restrict__ upd, const double* unum, const double* uden,
+      const double* fin_partial, unsigned fin_n, double* fin_result, int nrs, size_t rstride) {
+  dwf_body<T, OP, MAT, ADDX, OUT2, LSC, VEC>(
+      out, in, links, nbr, parity, Vh, Ls_rt, spb_rt, E, scale, X, xcoef, out2, F, G, X2, nrm,
+      xnum, xden, upd, unum, uden, fin_partial, fin_n, fin_result,
+      xcd_block(blockIdx.x, gridDim.x), gridDim.x, nrs, rstride);
+}
+
+// Arguments of one member of a grouped launch (one lattice, or several
+// right-hand sides of one lattice), read by every block from device memory.
+template <typename T> struct DwfArgs {
+  using C = typename St<T>::C;
+  T* out; const T* in; const T* links; const int* nbr;
+  const C* E; const T* X; T* out2; const C* F; const C* G; const T* X2;
+  double* nrm; const double* xnum; const double* xden;
+  T* upd; const double* unum; const double* uden;
+  const double* fin_partial; double* fin_result;
+  size_t rstride;
+  C scale, xcoef;
+  int parity, Vh, nrs, blk_begin, nblk;
+  unsigned fin_n;
+};
+
+// Lattices of different volume (same Ls, precision and epilogue form) in one
+// launch: block w (after the XCD remapping) belongs to the last member whose
+// blk_begin <= w; members are at most kGroupMax.
+template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC, int VEC>
+__global__ void __launch_bounds__(KCfg<T>::THREADS, (k_waves<T, OP>()))
+k_dwf_group(const DwfArgs<T>* __restrict__ tab, int n) {
+  const int w = xcd_block(blockIdx.x, gridDim.x);
+  int g = 0;
+#pragma unroll
+  for (int i = 1; i < kGroupMax; ++i)
+    if (i < n && w >= tab[i].blk_begin) g = i;
+  const DwfArgs<T>& a = tab[g];
+  dwf_body<T, OP, MAT, ADDX, OUT2, LSC, VEC>(
+      a.out, a.in, a.links, a.nbr, a.parity, a.Vh, LSC, 0, a.E, a.scale, a.X, a.xcoef, a.out2,
+      a.F, a.G, a.X2, a.nrm, a.xnum, a.xden, a.upd, a.unum, a.uden, a.fin_partial, a.fin_n,
+      a.fin_result, w - a.blk_begin, a.nblk, a.nrs, a.rstride);
+}
+
 template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC>
 static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* out,
                      const void* in, const void* links, const int* nbr, int parity, int Vh,
@@ -1141,7 +1224,7 @@ static void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, void* ou
                      (const T*)in, (const T*)links, nbr, parity, Vh, Ls, spb, (const C*)e.E,
                      (C)e.scale, (const T*)e.X, (C)e.xcoef, (T*)e.out2, (const C*)e.F,
                      (const C*)e.G, (const T*)e.X2, e.nrm_partial, e.xnum, e.xden, (T*)e.upd,
-                     e.unum, e.uden, e.fin_partial, e.fin_n, e.fin_result);
+                     e.unum, e.uden, e.fin_partial, e.fin_n, e.fin_result, e.nrs, e.rstride);
 }
 
 // the (op, epilogue) combinations the operator chains use
@@ -1219,9 +1302,10 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const bool mfma_path = mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap;
   if (e.upd && !(mfma_path && (op == K_SITE ? addx : !out2))) return hipErrorInvalidValue;
   if (!out && !e.upd && !(mfma_path && out2)) return hipErrorInvalidValue;
+  const int VhV = Vh << e.nrs;   // virtual sites (site, rhs)
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
-    const dim3 block(spb * (Ls / vec)), grid((Vh + spb - 1) / spb);
+    const dim3 block(spb * (Ls / vec)), grid((VhV + spb - 1) / spb);
     if (e.fin_partial && block.x % 64) return hipErrorInvalidValue;   // block_sum: whole waves
     *nblocks = grid.x;
     switch (Ls) {
@@ -1236,7 +1320,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   while (spb > 1 && lds_for(spb) > lds_cap) --spb;
   if (lds_for(spb) > lds_cap) return hipErrorInvalidValue;
   const size_t lds = lds_for(spb);
-  const dim3 block(spb * Ls), grid((Vh + spb - 1) / spb);
+  const dim3 block(spb * Ls), grid((VhV + spb - 1) / spb);
   if (e.fin_partial && block.x % 64) return hipErrorInvalidValue;
   *nblocks = grid.x;
   return launch_combo<T, 0>(op, mat, addx, out2, grid, block, lds, st, out, in, links, nbr,
@@ -1262,6 +1346,113 @@ hipError_t launch_dwf(int prec, int op, void* out, const void* in, const void* l
     err = hipGetLastError();
   }
   return err;
+}
+
+// ---------------------------------------------------------------------------
+// grouped launches (several lattices in one launch)
+// ---------------------------------------------------------------------------
+template <typename T, int OP, bool MAT, bool ADDX, bool OUT2, int LSC>
+static void launch_g(const GroupLaunch& gl, hipStream_t st) {
+  hipLaunchKernelGGL((k_dwf_group<T, OP, MAT, ADDX, OUT2, LSC, KCfg<T>::VEC>), dim3(gl.grid),
+                     dim3(gl.block), gl.lds, st, (const DwfArgs<T>*)gl.table, gl.n);
+}
+// the combinations the D^+D chain uses
+template <typename T, int LSC>
+static hipError_t launch_group_t(const GroupLaunch& gl, hipStream_t st) {
+  if (!gl.mat) return hipErrorInvalidValue;
+  if (gl.op == K_SITE && !gl.addx && !gl.out2) launch_g<T, K_SITE, true, false, false, LSC>(gl, st);
+  else if (gl.op == K_HOP && !gl.addx && !gl.out2) launch_g<T, K_HOP, true, false, false, LSC>(gl, st);
+  else if (gl.op == K_HOP && gl.addx && gl.out2) launch_g<T, K_HOP, true, true, true, LSC>(gl, st);
+  else if (gl.op == K_HOPDAG && !gl.addx && !gl.out2) launch_g<T, K_HOPDAG, true, false, false, LSC>(gl, st);
+  else if (gl.op == K_HOPDAG && gl.addx && !gl.out2) launch_g<T, K_HOPDAG, true, true, false, LSC>(gl, st);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t prepare_group_t(int op, int Ls, int n, const GroupMember* m, GroupLaunch* gl) {
+  using C = typename St<T>::C;
+  if (KCfg<T>::VEC != 1 && Ls % KCfg<T>::VEC) return hipErrorInvalidValue;
+  const int spb = KCfg<T>::THREADS / (Ls / KCfg<T>::VEC);
+  const Epi& e0 = m[0].e;
+  const bool mat = e0.E != nullptr, addx = e0.X != nullptr, out2 = e0.out2 != nullptr;
+  std::vector<DwfArgs<T>> tab(n);
+  int begin = 0;
+  for (int i = 0; i < n; ++i) {
+    const Epi& e = m[i].e;
+    if ((e.E != nullptr) != mat || (e.X != nullptr) != addx || (e.out2 != nullptr) != out2 ||
+        (e.G != nullptr) != (e0.G != nullptr) || (e.X2 != nullptr) != (e0.X2 != nullptr) ||
+        e.upd || e.xnum || e.nrm_partial || e.fin_partial)
+      return hipErrorInvalidValue;   // one epilogue form per launch; no fused CG updates
+    DwfArgs<T>& a = tab[i];
+    a.out = (T*)m[i].out;
+    a.in = (const T*)m[i].in;
+    a.links = (const T*)m[i].links;
+    a.nbr = m[i].nbr;
+    a.E = (const C*)e.E;
+    a.X = (const T*)e.X;
+    a.out2 = (T*)e.out2;
+    a.F = (const C*)e.F;
+    a.G = (const C*)e.G;
+    a.X2 = (const T*)e.X2;
+    a.nrm = nullptr;
+    a.xnum = a.xden = nullptr;
+    a.upd = nullptr;
+    a.unum = a.uden = nullptr;
+    a.fin_partial = nullptr;
+    a.fin_result = nullptr;
+    a.fin_n = 0;
+    a.rstride = e.rstride;
+    a.scale = (C)e.scale;
+    a.xcoef = (C)e.xcoef;
+    a.parity = m[i].parity;
+    a.Vh = m[i].Vh;
+    a.nrs = e.nrs;
+    a.nblk = ((m[i].Vh << e.nrs) + spb - 1) / spb;
+    a.blk_begin = begin;
+    begin += a.nblk;
+  }
+  const size_t cs = sizeof(C);
+  const size_t matb = 2 * (size_t)Ls * Ls * cs * (1 + (out2 ? 1 : 0) + (e0.G ? 1 : 0));
+  const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * spb * cs;
+  const size_t epi = (size_t)spb * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs;
+  gl->lds = (stage > epi ? stage : epi) + matb;
+  if (gl->lds > 64 * 1024) return hipErrorInvalidValue;
+  gl->op = op;
+  gl->Ls = Ls;
+  gl->n = n;
+  gl->mat = mat;
+  gl->addx = addx;
+  gl->out2 = out2;
+  gl->grid = begin;
+  gl->block = spb * (Ls / KCfg<T>::VEC);
+  hipError_t err = hipMalloc(&gl->table, sizeof(DwfArgs<T>) * n);
+  if (err != hipSuccess) return err;
+  return hipMemcpy(gl->table, tab.data(), sizeof(DwfArgs<T>) * n, hipMemcpyHostToDevice);
+}
+
+hipError_t prepare_dwf_group(int prec, int op, int Ls, int n, const GroupMember* m,
+                             GroupLaunch* gl) {
+  if (n < 1 || n > kGroupMax || (Ls != 12 && Ls != 16)) return hipErrorInvalidValue;
+  gl->prec = prec;
+  switch (prec) {
+    case P64: return prepare_group_t<double>(op, Ls, n, m, gl);
+    case P32: return prepare_group_t<float>(op, Ls, n, m, gl);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_dwf_group(const GroupLaunch& gl, hipStream_t st) {
+  if (gl.prec == P64) return gl.Ls == 12 ? launch_group_t<double, 12>(gl, st)
+                                         : launch_group_t<double, 16>(gl, st);
+  if (gl.prec == P32) return gl.Ls == 12 ? launch_group_t<float, 12>(gl, st)
+                                         : launch_group_t<float, 16>(gl, st);
+  return hipErrorInvalidValue;
+}
+
+void free_dwf_group(GroupLaunch* gl) {
+  if (gl->table) (void)hipFree(gl->table);
+  gl->table = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1483,6 +1674,16 @@ __global__ void __launch_bounds__(256) k_set_zero(T* __restrict__ x, size_t n) {
   TAIL_LOOP(n) St<T>::st(x + i, 0);
 }
 template <typename T>
+__global__ void __launch_bounds__(256) k_set_value(T* __restrict__ x, typename St<T>::C a,
+                                                   size_t n) {
+  using C = typename St<T>::C;
+  C z[Vec<T>::N];
+#pragma unroll
+  for (int j = 0; j < Vec<T>::N; ++j) z[j] = a;
+  PACK_LOOP(T, n) Vec<T>::st(x + ip * NV, z);
+  TAIL_LOOP(n) St<T>::st(x + i, a);
+}
+template <typename T>
 __global__ void __launch_bounds__(256) k_copy(T* __restrict__ y, const T* __restrict__ x, size_t n) {
   const size_t nb = n * sizeof(T) / 16;
   const uint4* xs = reinterpret_cast<const uint4*>(x);
@@ -1545,6 +1746,11 @@ static void setz_t(void* x, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_set_zero<T>, grid_for(n), dim3(256), 0, st, (T*)x, n);
 }
 template <typename T>
+static void setv_t(void* x, double a, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_value<T>, grid_for(n), dim3(256), 0, st, (T*)x,
+                     (typename St<T>::C)a, n);
+}
+template <typename T>
 static void copy_t(void* y, const void* x, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy<T>, grid_for(n), dim3(256), 0, st, (T*)y, (const T*)x, n);
 }
@@ -1565,6 +1771,10 @@ static void scal_t(void* x, double a, size_t n, hipStream_t st) {
 
 hipError_t launch_set_zero(int prec, void* x, size_t n, hipStream_t st) {
   DWF_DISPATCH(prec, setz_t, (x, n, st));
+  return hipGetLastError();
+}
+hipError_t launch_set_value(int prec, void* x, double a, size_t n, hipStream_t st) {
+  DWF_DISPATCH(prec, setv_t, (x, a, n, st));
   return hipGetLastError();
 }
 hipError_t launch_copy(int prec, void* y, const void* x, size_t n, hipStream_t st) {

@@ -64,6 +64,10 @@ class Field:
     def set_zero(self):
         _lib.check("dwf_field_set_zero", _lib.load().dwf_field_set_zero(self.fopr._h, self._p))
 
+    def set(self, a):
+        """AField::set(a): every stored element equals a."""
+        _lib.check("dwf_field_set", _lib.load().dwf_field_set(self.fopr._h, self._p, float(a)))
+
     def copy_from(self, other):
         _lib.check("dwf_field_copy", _lib.load().dwf_field_copy(self.fopr._h, self._p, other._p))
 
@@ -399,6 +403,51 @@ class AShiftsolver_CG:
         _lib.check("dwf_shiftsolver_cg_solve", rc,
                    allow=() if raise_on_nonconvergence else (_lib.ERR_NOT_CONVERGED,))
         return n.value, float(d[0])
+
+
+class AShiftsolver_CG_Batch:
+    """Batched multishift CG: several right-hand sides (on one operator or on
+    up to four lattices) with the same shifts advance together; every hop of
+    D^+D is one launch for all of them and the scalar recurrences of
+    AShiftsolver_CG stay on the device.  graph=True replays the iteration from
+    a captured HIP graph."""
+
+    def __init__(self, fopr_list, nshift, graph=False):
+        self.fopr_list = list(fopr_list)
+        self.nshift = int(nshift)
+        ops = (C.c_void_p * len(self.fopr_list))(*[f._h for f in self.fopr_list])
+        h = C.c_void_p()
+        _lib.check("dwf_msbatch_create", _lib.load().dwf_msbatch_create(
+            len(self.fopr_list), ops, self.nshift, 1 if graph else 0, C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.load().dwf_msbatch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def solve(self, xs, sigma, bs, maximum_number_of_iteration, convergence_criterion_squared,
+              raise_on_nonconvergence=True):
+        """xs[k][j] solves (A_k + sigma_j) x = bs[k]; returns (nconv list, diff list)."""
+        nsys = len(self.fopr_list)
+        assert len(bs) == nsys and len(xs) == nsys and len(sigma) == self.nshift
+        sig = np.ascontiguousarray(sigma, np.float64)
+        xa = (C.c_void_p * (nsys * self.nshift))(*[x.ptr for row in xs for x in row])
+        ba = (C.c_void_p * nsys)(*[b.ptr for b in bs])
+        n = (C.c_int * nsys)()
+        d = np.zeros(nsys)
+        rc = _lib.load().dwf_msbatch_solve(self._h, _ptr(sig), xa, ba,
+                                           int(maximum_number_of_iteration),
+                                           float(convergence_criterion_squared), n, _ptr(d))
+        _lib.check("dwf_msbatch_solve", rc,
+                   allow=() if raise_on_nonconvergence else (_lib.ERR_NOT_CONVERGED,))
+        return [int(v) for v in n], [float(v) for v in d]
+
+    def iterations(self):
+        return int(_lib.load().dwf_msbatch_iterations(self._h))
 
 
 class DeviceArray:

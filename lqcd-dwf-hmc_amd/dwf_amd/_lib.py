@@ -86,6 +86,7 @@ _SIGS = {
     "dwf_field_alloc": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dwf_field_free": (C.c_int, [_vp]),
     "dwf_field_set_zero": (C.c_int, [_vp, _vp]),
+    "dwf_field_set": (C.c_int, [_vp, _vp, C.c_double]),
     "dwf_field_copy": (C.c_int, [_vp, _vp, _vp]),
     "dwf_field_axpy": (C.c_int, [_vp, _vp, C.c_double, _vp]),
     "dwf_field_aypx": (C.c_int, [_vp, C.c_double, _vp, _vp]),
@@ -114,6 +115,12 @@ _SIGS = {
                                                    C.c_double, C.POINTER(_vp)]),
     "dwf_shiftsolver_cg_solve": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _dp,
                                            C.POINTER(_vp), _vp, _ip, _dp]),
+    "dwf_msbatch_create": (C.c_int, [C.c_int, C.POINTER(_vp), C.c_int, C.c_int,
+                                     C.POINTER(_vp)]),
+    "dwf_msbatch_solve": (C.c_int, [_vp, _dp, C.POINTER(_vp), C.POINTER(_vp), C.c_int,
+                                    C.c_double, _ip, _dp]),
+    "dwf_msbatch_iterations": (C.c_int, [_vp]),
+    "dwf_msbatch_destroy": (C.c_int, [_vp]),
     "dwf_force_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dwf_force_destroy": (C.c_int, [_vp]),
     "dwf_force_set_mode": (C.c_int, [_vp, C.c_char_p]),

@@ -66,10 +66,8 @@ class AField {
   dwf_fopr fopr() const { return op_; }
 
   // BLAS-1 of AField (reductions in double)
-  void set(double zero) {
-    if (zero != 0.0) throw std::invalid_argument("AField::set supports 0 only");
-    check(dwf_field_set_zero(op_, p_), "dwf_field_set_zero");
-  }
+  // AField::set(a): every element equals a (afield.h:158)
+  void set(double a) { check(dwf_field_set(op_, p_, a), "dwf_field_set"); }
   double norm2() const {
     double r;
     check(dwf_field_norm2(op_, p_, &r), "dwf_field_norm2");

@@ -23,6 +23,9 @@ PARAMS = [
 ]
 LS16 = [((4, 4, 4, 8), 16, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1)),      # matrix-core epilogue
         ((4, 2, 4, 4), 16, 0.05, 1.0, 1.5, 0.5, (1, 1, 1, -1))]
+# Ls = 12 (the reference benchmark's Ls): compile-time Ls with a partial last
+# wave per block (42 sites x 6 threads at 16-bit storage)
+LS12 = [((4, 4, 4, 8), 12, 0.02, 1.6, 1.5, 0.5, (1, 1, 1, -1))]
 MODES = ["D", "Ddag", "DdagD", "DDdag", "H", "Hdag", "Deo", "Doe", "Dee", "Doo",
          "Dee_inv", "Doo_inv"]
 
@@ -41,7 +44,7 @@ def pair(prm, precision="double", seed=5, eps=0.5):
     return orc, gpu, U
 
 
-@pytest.mark.parametrize("prm", PARAMS + LS16[1:])
+@pytest.mark.parametrize("prm", PARAMS + LS16[1:] + LS12)
 def test_modes_fp64(prm):
     print("case", prm, flush=True)
     orc, gpu, U = pair(prm)
@@ -53,7 +56,7 @@ def test_modes_fp64(prm):
             assert rel(got, want) < 1e-13, (mode, dag, rel(got, want))
 
 
-@pytest.mark.parametrize("prm", PARAMS[:2] + LS16)
+@pytest.mark.parametrize("prm", PARAMS[:2] + LS16 + LS12)
 @pytest.mark.parametrize("precision,tol", [("float", 2e-6), ("half", 4e-3), ("bfloat16", 3e-2)])
 def test_modes_reduced_precision(prm, precision, tol):
     orc, gpu, U = pair(prm, precision)
@@ -62,6 +65,40 @@ def test_modes_reduced_precision(prm, precision, tol):
         want = orc.mult(mode, x)
         got = gpu.apply(mode, x)
         assert rel(got, want) < tol, (precision, mode, rel(got, want))
+
+
+@pytest.mark.parametrize("prm", [PARAMS[1], LS16[1]], ids=["ls6", "ls16"])
+def test_modes_fp64_alpha(prm):
+    """The Accel alpha (Neff) form, parameter_alpha = 1.3, against the oracle's
+    f[0] = mq dm[0] / alpha construction, every mode incl. Dee_inv / Doo_inv."""
+    dims, Ls, mq, M0, b, c, bc = prm
+    U = lattice.random_gauge(dims, 0.5, 31)
+    orc = OracleDWF(dims, Ls, mq, M0, b, c, alpha=1.3, bc=bc)
+    orc.set_config(U)
+    gpu = dwf_amd.AFopr_Domainwall_5din_eo(dims, Ls, mq, M0, b, c, bc, "double",
+                                           parameter_alpha=1.3)
+    gpu.set_config(U)
+    x = lattice.random_spinor5(dims, Ls, 32)
+    for mode in MODES:
+        for dag in (False, True):
+            want = orc.mult_dag(mode, x) if dag else orc.mult(mode, x)
+            got = gpu.apply(mode, x, dagger=dag)
+            assert rel(got, want) < 1e-13, (mode, dag, rel(got, want))
+
+
+def test_richardson_fp32_inner_ls12():
+    prm = LS12[0]
+    orc, gpu, U = pair(prm)
+    gf = dwf_amd.AFopr_Domainwall_5din_eo(*prm, "float")
+    gf.set_config(U)
+    b = lattice.random_spinor5(prm[0], prm[1], 33)
+    gpu.set_mode("DdagD")
+    fb, fx = gpu.new_field(), gpu.new_field()
+    gpu.convert(fb, b)
+    n, d = dwf_amd.ASolver_Richardson(gpu, gf, 100, 1e-24, 10000, 1e-8).solve(fx, fb)
+    x = gpu.reverse(fx)
+    true = np.linalg.norm(orc.mult("DdagD", x) - b) / np.linalg.norm(b)
+    assert n >= 2 and d < 1e-12 and true < 1e-11, (n, d, true)
 
 
 def test_gm5_and_reflection():
@@ -115,6 +152,8 @@ def test_blas():
     assert np.array_equal(gpu.reverse(fz), gpu.reverse(fy))
     fz.set_zero()
     assert fz.norm2() == 0.0
+    fz.set(0.25)   # AField::set(a): every element equals a
+    assert np.array_equal(gpu.reverse(fz), np.full_like(x, 0.25))
 
 
 def test_cg_fp64_matches_oracle():
