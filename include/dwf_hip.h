@@ -92,6 +92,15 @@ const char* dwf_fopr_get_mode(dwf_fopr op);
 /* mult(v, w) / mult_dag(v, w) in the current mode; v, w device fields. */
 int dwf_fopr_mult(dwf_fopr op, void* v, const void* w);
 int dwf_fopr_mult_dag(dwf_fopr op, void* v, const void* w);
+/* mult(v, w) in the current mode without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The intermediate fields live in a
+ * static per-GPU scratch array of 256 MiB (5 fields of the operator, each
+ * rounded up to 256 bytes, must fit: e.g. FP16 storage at 16^3 x 32, Ls = 16);
+ * larger operators are rejected with DWF_ERR_INVALID_ARG.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race.  The library must
+ * be initialised (an operator created) on the current device. */
+int dwf_fopr_mult_noalloc(dwf_fopr op, void* v, const void* w);
 /* mult(v, w, mode) / mult_dag(v, w, mode) (afopr_Domainwall_eo-tmpl.h:475-525) */
 int dwf_fopr_mult_mode(dwf_fopr op, void* v, const void* w, const char* mode);
 int dwf_fopr_mult_dag_mode(dwf_fopr op, void* v, const void* w, const char* mode);

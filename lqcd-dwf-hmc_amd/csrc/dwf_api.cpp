@@ -65,6 +65,27 @@ struct Ctx {
 static Ctx g_ctx;
 static std::mutex g_mtx;
 
+// Scratch of dwf_fopr_mult_noalloc: the operator's intermediate fields for
+// callers that cannot allocate.  One copy per GPU, held until the process
+// ends; host code uses the per-device address from hipGetSymbolAddress, never
+// the array's name.
+constexpr size_t kNoAllocScratch = (size_t)256 << 20;   // 256 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// address of this device's scratch array (the current device must be dev)
+static int scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(DWF_ERR_NO_DEVICE, "device index out of range");
+  if (g_scratch_ptr[dev]) return DWF_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(DWF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = ptr;
+  return DWF_OK;
+}
+
 static int ctx_init() {
   if (g_ctx.ok) return DWF_OK;
   std::lock_guard<std::mutex> lk(g_mtx);
@@ -81,6 +102,10 @@ static int ctx_init() {
   HIPCHK(hipHostMalloc(&g_ctx.host_result, sizeof(double) * 64, hipHostMallocDefault));
   for (auto& e : g_ctx.ring_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipHostMalloc(&g_ctx.host_ring, sizeof(double) * 8, hipHostMallocCoherent));
+  {
+    const int rc = scratch_init(g_ctx.device);
+    if (rc != DWF_OK) return rc;
+  }
   g_ctx.ok = true;
   return DWF_OK;
 }
@@ -765,6 +790,33 @@ int dwf_fopr_mult(dwf_fopr op, void* v, const void* w) {
   CTX();
   if (!op) return fail(DWF_ERR_INVALID_ARG, "null operator");
   return mult_impl(op, v, w, op->mode, false);
+}
+// mult(v, w) with the intermediate fields in the library's static scratch
+// (5 fields of 256-byte aligned slices): no allocation.  Calls that overlap on
+// one GPU share the scratch and race.
+int dwf_fopr_mult_noalloc(dwf_fopr op, void* v, const void* w) {
+  if (!g_ctx.ok) return fail(DWF_ERR_NO_DEVICE, "library not initialised (create an operator first)");
+  if (!op) return fail(DWF_ERR_INVALID_ARG, "null operator");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != g_ctx.device)
+    return fail(DWF_ERR_INVALID_ARG, "the current device is not the library's device");
+  RC(scratch_init(dev));
+  const size_t fb = (op->nelem * prec_bytes(op->prec) + 255) & ~(size_t)255;
+  if (5 * fb > kNoAllocScratch)
+    return fail(DWF_ERR_INVALID_ARG,
+                "dwf_fopr_mult_noalloc: needs 5 fields of " + std::to_string(fb) +
+                    " bytes of scratch, the maximum is " + std::to_string(kNoAllocScratch) +
+                    " bytes in total");
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  void* saved[5];
+  for (int i = 0; i < 5; ++i) {
+    saved[i] = op->work[i];
+    op->work[i] = base + i * fb;
+  }
+  const int rc = mult_impl(op, v, w, op->mode, false);
+  for (int i = 0; i < 5; ++i) op->work[i] = saved[i];
+  return rc;
 }
 int dwf_fopr_mult_dag(dwf_fopr op, void* v, const void* w) {
   CTX();

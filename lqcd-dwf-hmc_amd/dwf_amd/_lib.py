@@ -69,6 +69,7 @@ _SIGS = {
     "dwf_fopr_get_mode": (C.c_char_p, [_vp]),
     "dwf_fopr_mult": (C.c_int, [_vp, _vp, _vp]),
     "dwf_fopr_mult_dag": (C.c_int, [_vp, _vp, _vp]),
+    "dwf_fopr_mult_noalloc": (C.c_int, [_vp, _vp, _vp]),
     "dwf_fopr_mult_mode": (C.c_int, [_vp, _vp, _vp, C.c_char_p]),
     "dwf_fopr_mult_dag_mode": (C.c_int, [_vp, _vp, _vp, C.c_char_p]),
     "dwf_fopr_mult_gm5": (C.c_int, [_vp, _vp, _vp]),
