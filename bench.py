@@ -116,6 +116,90 @@ def cpu_baseline(U_smeared, x, seconds):
     }
 
 
+def bench_batch(dwf_amd, lattice, np, f64, f32, sm, log):
+    """Batched multishift (16 shifts log-spaced on [1e-4, 10], |r|^2/|b|^2 < 1e-20):
+    4 Gaussian right-hand sides on 16^3x32, Ls=16 batched against one multishift
+    per right-hand side, with and without graph replay, and in mixed precision;
+    then 3 lattices (8^4, 12^3x24, 16^3x32; Ls=12) in grouped launches against
+    one solve per lattice."""
+    sigma = np.logspace(-4, 1, 16)
+    nrhs = 4
+    out = {}
+
+    def run(fn):
+        fn()   # warm: allocations, graph capture
+        dwf_amd.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        dwf_amd.synchronize()
+        return time.perf_counter() - t0, r
+
+    bs = []
+    for k in range(nrhs):
+        fb = f64.new_field()
+        f64.convert(fb, lattice.random_spinor5(DIMS, LS, 300 + k))
+        bs.append(fb)
+    xs = [[f64.new_field() for _ in sigma] for _ in range(nrhs)]
+    single = dwf_amd.AShiftsolver_CG(f64, 100000, TOL2)
+
+    def seq():
+        return [single.solve(xs[k], sigma, bs[k]) for k in range(nrhs)]
+    t_seq, r_seq = run(seq)
+    log(f"multishift 4 RHS sequential: {t_seq:.3f} s, iterations {[r[0] for r in r_seq]}")
+    out["sequential_s"] = round(t_seq, 4)
+    for name, graph in (("batched", False), ("batched_graph", True)):
+        solver = dwf_amd.AShiftsolver_CG_Batch([f64] * nrhs, len(sigma), graph=graph)
+        t_b, (ns, ds) = run(lambda: solver.solve(xs, sigma, bs, 100000, TOL2))
+        if max(ds) >= 1e-10 or min(ns) < 0:
+            raise RuntimeError(f"batched multishift missed the target: {ns} {ds}")
+        log(f"multishift 4 RHS {name}: {t_b:.3f} s, iterations {ns}, max diff {max(ds):.2e}")
+        out[name + "_s"] = round(t_b, 4)
+        out[name + "_iterations"] = solver.iterations()
+        out[name + "_ms_per_iteration"] = round(t_b / solver.iterations() * 1e3, 3)
+        del solver
+    out["speedup_batched_vs_sequential"] = round(t_seq / out["batched_s"], 3)
+    out["speedup_graph_vs_sequential"] = round(t_seq / out["batched_graph_s"], 3)
+    # mixed precision: FP32 batched multishift, every shift refined in FP64
+    mixed = dwf_amd.AShiftsolver_CG_Batch([f32] * nrhs, len(sigma))
+    t_m, (ns, ds) = run(lambda: mixed.solve_mixed([f64] * nrhs, xs, sigma, bs, 100000, TOL2,
+                                                  low=1e-10, inner=1e-10))
+    if max(ds) >= 1e-10:
+        raise RuntimeError(f"mixed batched multishift missed the target: {ds}")
+    log(f"multishift 4 RHS mixed FP32+FP64 refinement: {t_m:.3f} s, FP32 iterations {ns}, "
+        f"refinement CG iterations {mixed.refine_iterations()}, max diff {max(ds):.2e}")
+    out["mixed_fp32_refined_s"] = round(t_m, 4)
+    out["mixed_refinement_cg_iterations"] = mixed.refine_iterations()
+    out["max_diff_mixed"] = float(f"{max(ds):.3e}")
+    del mixed, xs, bs, single
+
+    # three lattices of uneven volume, Ls = 12, one right-hand side each
+    lats = [(8, 8, 8, 8), (12, 12, 12, 24), (16, 16, 16, 32)]
+    ops, bl, xl = [], [], []
+    for i, d in enumerate(lats):
+        U = lattice.random_gauge(d, EPS, 400 + i)
+        smi = dwf_amd.Director_alt_Smear(d, RHO, NSMEAR)
+        op = dwf_amd.AFopr_Domainwall_5din_eo(d, 12, MQ, M0, B, CC, (1, 1, 1, -1), "double")
+        op.set_config_smeared(smi, U)
+        op.set_mode("DdagD")
+        fb = op.new_field()
+        op.convert(fb, lattice.random_spinor5(d, 12, 410 + i))
+        ops.append(op)
+        bl.append(fb)
+        xl.append([op.new_field() for _ in sigma])
+    singles = [dwf_amd.AShiftsolver_CG(op, 100000, TOL2) for op in ops]
+    t_s3, r3 = run(lambda: [singles[i].solve(xl[i], sigma, bl[i]) for i in range(3)])
+    grouped = dwf_amd.AShiftsolver_CG_Batch(ops, len(sigma), graph=True)
+    t_g3, (ns, ds) = run(lambda: grouped.solve(xl, sigma, bl, 100000, TOL2))
+    if max(ds) >= 1e-10 or min(ns) < 0:
+        raise RuntimeError(f"grouped multishift missed the target: {ns} {ds}")
+    log(f"3 lattices Ls=12: sequential {t_s3:.3f} s, grouped+graph {t_g3:.3f} s, "
+        f"iterations {ns}")
+    out["three_lattices_ls12"] = {"sequential_s": round(t_s3, 4), "grouped_graph_s": round(t_g3, 4),
+                                  "iterations": ns,
+                                  "speedup": round(t_s3 / t_g3, 3)}
+    return out
+
+
 def load_traffic(key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -137,6 +221,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-cg", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (the lattice "
@@ -297,6 +382,10 @@ def main():
     del xs
     log(f"multishift CG, 16 shifts: {t_ms:.3f} s, {n_ms} iterations, max rel residual {d_ms:.2e}")
 
+    batch = None
+    if not a.no_batch:
+        batch = bench_batch(dwf_amd, lattice, np, f64, f32, sm, log)
+
     cpu = None
     if not a.no_cpu_baseline:
         Us = sm.get_config(NSMEAR)
@@ -345,6 +434,7 @@ def main():
         "multishift_16_shifts_fp64": {"time_s": round(t_ms, 4), "iterations": n_ms,
                                       "ms_per_iteration": round(t_ms / max(n_ms, 1) * 1e3, 3),
                                       "max_rel_residual": d_ms},
+        "multishift_16_shifts_4_rhs_batched": batch,
         "operator_rel_error_vs_fp64": {k: float(f"{e:.3e}") for k, e in op_err.items()},
         # the hop the timed solve spends its time in: FP16 storage D_eo, algorithmic
         # bytes (16-bit spinors and links) per launch over its HIP-event time

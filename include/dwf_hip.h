@@ -200,6 +200,21 @@ int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* 
 /* iterations of the last solve (the largest over the systems) */
 int dwf_msbatch_iterations(dwf_msbatch m);
 int dwf_msbatch_destroy(dwf_msbatch m);
+/* Mixed precision (m created on FP32 operators, ops64[k] the FP64 operator of
+ * system k on the same lattice, mode DdagD): the batched multishift in FP32 to
+ * stop_cond_low, then every (system, shift) refined in FP64 with the
+ * Richardson / Mixedprec scheme (asolver_Richardson-tmpl.h:131-213,
+ * aprecond_Mixedprec-tmpl.h:55-108) on the shifted operator: the residual
+ * b - (A + sigma_j) x in FP64, the correction by CG on the FP32 operator
+ * (A + sigma_j) to stop_cond_inner, until |r|^2/|b|^2 < stop_cond.  x and b
+ * are fields of ops64; nconv[k] = iterations of the FP32 multishift;
+ * diff[k] = max_j |(A + sigma_j) x_kj - b_k| / |b_k| in FP64. */
+int dwf_msbatch_solve_mixed(dwf_msbatch m, const dwf_fopr* ops64, const double* sigma, void** x,
+                            const void* const* b, int max_iter, double stop_cond,
+                            double stop_cond_low, double stop_cond_inner, int* nconv,
+                            double* diff);
+/* CG iterations of the last mixed solve's refinement, summed */
+int dwf_msbatch_refine_iterations(dwf_msbatch m);
 
 /* ------------------------------------------------------------------------
  * Learned polynomial preconditioner (no counterpart in the reference; it is a

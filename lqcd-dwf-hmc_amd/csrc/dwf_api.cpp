@@ -2035,6 +2035,10 @@ struct dwf_msbatch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int iters_total = 0;
+  // mixed precision: low-precision sources / solutions and CG work fields
+  std::vector<void*> bl, xl;
+  void* lw[4] = {};
+  int refine_iters = 0;
 };
 
 // the D^+D chain of op_D (with v2) + op_Ddag (with pre) on the member buffers
@@ -2132,6 +2136,12 @@ static void ms_release(dwf_msbatch_s* m) {
   if (m->L.partial) (void)hipFree(m->L.partial);
   if (m->L.pp_res) (void)hipFree(m->L.pp_res);
   if (m->hstatus) (void)hipHostFree(m->hstatus);
+  for (auto& q : m->bl)
+    if (q) (void)hipFree(q);
+  for (auto& q : m->xl)
+    if (q) (void)hipFree(q);
+  for (auto& q : m->lw)
+    if (q) (void)hipFree(q);
 }
 
 extern "C" {
@@ -2264,6 +2274,132 @@ int dwf_msbatch_destroy(dwf_msbatch m) {
 }
 
 int dwf_msbatch_iterations(dwf_msbatch m) { return m ? m->iters_total : -1; }
+
+}  // extern "C"
+
+// CG on (A + sigma) e = r of op (mode DdagD, any precision) from e = 0 to
+// |r|^2/|b|^2 < tol (ASolver_CG's iteration with the shift added to the
+// operator); t1, t2, t3: work fields of op.  Returns the iterations done.
+static int shifted_cg(dwf_fopr_s* op, double sigma, void* e, const void* b, double tol,
+                      int max_iter, void* r, void* p, void* s, int* iters) {
+  hipStream_t st = g_ctx.stream;
+  const size_t n = op->nelem;
+  double bb;
+  HIPCHK(launch_norm2(op->prec, b, n, g_ctx.red, 0, st));
+  RC(fetch(1, &bb));
+  HIPCHK(launch_set_zero(op->prec, e, n, st));
+  HIPCHK(launch_copy(op->prec, r, b, n, st));
+  HIPCHK(launch_copy(op->prec, p, b, n, st));
+  double rr = bb;
+  int it = 0;
+  for (; it < max_iter && rr > tol * bb; ++it) {
+    RC(mult_impl(op, s, p, "DdagD", false));
+    double pa;
+    HIPCHK(launch_axpy_dot(op->prec, s, sigma, p, n, g_ctx.red, 0, st));
+    RC(fetch(1, &pa));
+    const double a = rr / pa;
+    double rr_new;
+    HIPCHK(launch_cg_xr(op->prec, e, r, p, s, a, n, g_ctx.red, 0, st));
+    RC(fetch(1, &rr_new));
+    HIPCHK(launch_aypx(op->prec, rr_new / rr, p, r, n, st));
+    rr = rr_new;
+  }
+  *iters = it;
+  return DWF_OK;
+}
+
+extern "C" {
+
+// Mixed precision: the batched multishift on the low-precision operators of m
+// to stop_cond_low, then every (system, shift) refined in FP64 by the
+// Richardson / Mixedprec scheme of asolver_Richardson-tmpl.h:131-213 and
+// aprecond_Mixedprec-tmpl.h:55-108 on the shifted operator: r = b - (A + s) x
+// in FP64, e = (A + s)^-1 (r / |r|) by CG on the low-precision operator to
+// stop_cond_inner, x += |r| e, until |r|^2 / |b|^2 < stop_cond.
+int dwf_msbatch_solve_mixed(dwf_msbatch m, const dwf_fopr* ops64, const double* sigma, void** x,
+                            const void* const* b, int max_iter, double stop_cond,
+                            double stop_cond_low, double stop_cond_inner, int* nconv,
+                            double* diff) {
+  CTX();
+  if (!m || !ops64 || !sigma || !x || !b || !nconv || !diff)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (m->prec != P32) return fail(DWF_ERR_UNSUPPORTED, "the batched solver must run in FP32");
+  const int nsys = m->nsys, nshift = m->nshift;
+  hipStream_t st = g_ctx.stream;
+  for (int k = 0; k < nsys; ++k) {
+    dwf_fopr_s* o = ops64[k];
+    if (!o || o->prec != P64 || o->nelem != m->ops[k]->nelem || o->mode != "DdagD")
+      return fail(DWF_ERR_INVALID_ARG, "ops64[k]: the FP64 operator (mode DdagD) of system k");
+  }
+  // low-precision sources and solutions, kept by the solver object
+  if (m->xl.empty()) {
+    m->bl.assign(nsys, nullptr);
+    m->xl.assign((size_t)nsys * nshift, nullptr);
+    for (int k = 0; k < nsys; ++k) {
+      const size_t fb = m->ops[k]->nelem * prec_bytes(P32);
+      HIPCHK(hipMalloc(&m->bl[k], fb));
+      for (int j = 0; j < nshift; ++j) HIPCHK(hipMalloc(&m->xl[k * nshift + j], fb));
+    }
+  }
+  for (int k = 0; k < nsys; ++k)
+    HIPCHK(launch_convert(P32, m->bl[k], P64, b[k], m->ops[k]->nelem, st));
+  std::vector<int> nc(nsys);
+  std::vector<double> dl(nsys);
+  const int rc = dwf_msbatch_solve(m, sigma, m->xl.data(), (const void* const*)m->bl.data(),
+                                   max_iter, stop_cond_low, nc.data(), dl.data());
+  if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) return rc;
+  m->refine_iters = 0;
+  bool all = true;
+  for (int k = 0; k < nsys; ++k) {
+    dwf_fopr_s* o = ops64[k];
+    dwf_fopr_s* ol = m->ops[k];
+    const size_t n = o->nelem;
+    void *t64, *rl, *el, *w1, *w2, *w3;
+    RC(get_work(o, 4, &t64));
+    RC(get_work(ol, 4, &rl));
+    if (!m->lw[0]) {
+      size_t mx = 0;
+      for (auto* q : m->ops) mx = std::max(mx, q->nelem);
+      for (auto& q : m->lw) HIPCHK(hipMalloc(&q, mx * prec_bytes(P32)));
+    }
+    el = m->lw[0];
+    w1 = m->lw[1];
+    w2 = m->lw[2];
+    w3 = m->lw[3];
+    double bn;
+    HIPCHK(launch_norm2(P64, b[k], n, g_ctx.red, 0, st));
+    RC(fetch(1, &bn));
+    double dmax = 0.0;
+    for (int j = 0; j < nshift; ++j) {
+      void* xj = x[k * nshift + j];
+      HIPCHK(launch_convert(P64, xj, P32, m->xl[k * nshift + j], n, st));
+      double rr = 0;
+      int outer = 0;
+      for (;; ++outer) {
+        // r = b - (A + sigma_j) x  (as (A + s) x - b, norm, then scaled by -1/|r|)
+        RC(mult_impl(o, t64, xj, "DdagD", false));
+        HIPCHK(launch_axpy(P64, t64, sigma[j], xj, n, st));
+        HIPCHK(launch_sub_norm2(P64, t64, t64, b[k], n, g_ctx.red, 0, st));
+        RC(fetch(1, &rr));
+        if (rr < stop_cond * bn || outer >= 100) break;
+        const double rn = std::sqrt(rr);
+        HIPCHK(launch_convert_scaled(P32, rl, P64, t64, -1.0 / rn, n, st));
+        int it = 0;
+        RC(shifted_cg(ol, sigma[j], el, rl, stop_cond_inner, 100000, w1, w2, w3, &it));
+        m->refine_iters += it;
+        HIPCHK(launch_axpy_from(P64, xj, rn, P32, el, n, st));
+      }
+      const double d1 = std::sqrt(rr / bn);
+      if (!(rr < stop_cond * bn)) all = false;
+      dmax = std::max(dmax, d1);
+    }
+    diff[k] = dmax;
+    nconv[k] = nc[k];
+  }
+  return all ? DWF_OK : fail(DWF_ERR_NOT_CONVERGED, "mixed batched multishift: not converged");
+}
+
+int dwf_msbatch_refine_iterations(dwf_msbatch m) { return m ? m->refine_iters : -1; }
 
 int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* const* b,
                       int max_iter, double stop_cond, int* nconv, double* diff) {

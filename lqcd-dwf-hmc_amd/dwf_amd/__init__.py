@@ -449,6 +449,30 @@ class AShiftsolver_CG_Batch:
     def iterations(self):
         return int(_lib.load().dwf_msbatch_iterations(self._h))
 
+    def solve_mixed(self, fopr64_list, xs, sigma, bs, maximum_number_of_iteration,
+                    convergence_criterion_squared, low=1e-10, inner=1e-10,
+                    raise_on_nonconvergence=True):
+        """FP32 batched multishift (this object, on FP32 operators) to |r|^2/|b|^2 < low,
+        then every shift refined in FP64 (xs, bs: fields of the FP64 operators)."""
+        nsys = len(self.fopr_list)
+        assert len(fopr64_list) == nsys and len(bs) == nsys and len(xs) == nsys
+        sig = np.ascontiguousarray(sigma, np.float64)
+        ops = (C.c_void_p * nsys)(*[f._h for f in fopr64_list])
+        xa = (C.c_void_p * (nsys * self.nshift))(*[x.ptr for row in xs for x in row])
+        ba = (C.c_void_p * nsys)(*[b.ptr for b in bs])
+        n = (C.c_int * nsys)()
+        d = np.zeros(nsys)
+        rc = _lib.load().dwf_msbatch_solve_mixed(self._h, ops, _ptr(sig), xa, ba,
+                                                 int(maximum_number_of_iteration),
+                                                 float(convergence_criterion_squared),
+                                                 float(low), float(inner), n, _ptr(d))
+        _lib.check("dwf_msbatch_solve_mixed", rc,
+                   allow=() if raise_on_nonconvergence else (_lib.ERR_NOT_CONVERGED,))
+        return [int(v) for v in n], [float(v) for v in d]
+
+    def refine_iterations(self):
+        return int(_lib.load().dwf_msbatch_refine_iterations(self._h))
+
 
 class DeviceArray:
     """A plain device buffer of float64 (e.g. a Field_G force field)."""

@@ -121,6 +121,10 @@ _SIGS = {
     "dwf_msbatch_solve": (C.c_int, [_vp, _dp, C.POINTER(_vp), C.POINTER(_vp), C.c_int,
                                     C.c_double, _ip, _dp]),
     "dwf_msbatch_iterations": (C.c_int, [_vp]),
+    "dwf_msbatch_solve_mixed": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.POINTER(_vp),
+                                          C.POINTER(_vp), C.c_int, C.c_double, C.c_double,
+                                          C.c_double, _ip, _dp]),
+    "dwf_msbatch_refine_iterations": (C.c_int, [_vp]),
     "dwf_msbatch_destroy": (C.c_int, [_vp]),
     "dwf_force_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dwf_force_destroy": (C.c_int, [_vp]),

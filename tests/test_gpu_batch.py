@@ -147,3 +147,37 @@ def test_batch_nonconvergence_reports_minus_one():
     ns, ds = dwf_amd.AShiftsolver_CG_Batch([gpu], 2).solve(xs, sigma, [b], 3, 1e-30,
                                                            raise_on_nonconvergence=False)
     assert ns == [-1] and ds[0] > 1e-4
+
+
+def test_batch_mixed_precision_refined_to_fp64():
+    """FP32 batched multishift, then every shift refined in FP64 (Richardson
+    with a shifted FP32 CG): FP64 true residuals below 1e-10."""
+    prm = ((4, 4, 4, 8), 16, 0.05, 1.6, 1.0, 0.0)
+    dims, Ls = prm[0], prm[1]
+    U = lattice.random_gauge(dims, 0.5, 99)
+    orc = OracleDWF(dims, Ls, *prm[2:])
+    orc.set_config(U)
+    g64 = dwf_amd.AFopr_Domainwall_5din_eo(dims, Ls, *prm[2:], (1, 1, 1, -1), "double")
+    g32 = dwf_amd.AFopr_Domainwall_5din_eo(dims, Ls, *prm[2:], (1, 1, 1, -1), "float")
+    for g in (g64, g32):
+        g.set_config(U)
+        g.set_mode("DdagD")
+    sigma = lattice.log_spaced_shifts(5, 1e-3, 10.0)
+    bs = [lattice.random_spinor5(dims, Ls, 100 + k) for k in range(2)]
+    fbs = []
+    for b in bs:
+        f = g64.new_field()
+        g64.convert(f, b)
+        fbs.append(f)
+    xs = [[g64.new_field() for _ in sigma] for _ in bs]
+    solver = dwf_amd.AShiftsolver_CG_Batch([g32] * 2, len(sigma))
+    ns, ds = solver.solve_mixed([g64] * 2, xs, sigma, fbs, 5000, 1e-22)
+    print("  refine iterations", solver.refine_iterations(), "diff", ds, flush=True)
+    for k in range(2):
+        assert ds[k] < 1e-11
+        xo, no, do, rc = orc.shift_cg(bs[k], sigma, "DdagD", niter=5000, stop_cond2=1e-24)
+        for s_, fx, xr in zip(sigma, xs[k], xo):
+            x = g64.reverse(fx)
+            r = orc.mult("DdagD", x) + s_ * x - bs[k]
+            assert np.linalg.norm(r) / np.linalg.norm(bs[k]) < 1e-11
+            assert rel(x, xr) < 1e-8
