@@ -369,6 +369,53 @@ int dwf_smear_force_udiv(dwf_smear sm, int level, double* Sigma, const double* S
 int dwf_smear_mult_jacobian(dwf_smear sm, double* F);
 int dwf_smear_mult_generator(dwf_smear sm, double* F);
 
+/* ------------------------------------------------------------------------
+ * Random numbers of the HMC (host; no device needed):
+ * RandomNumbers_parallel<RandomNumbers_Mseries>
+ * (src/lib/Tools/randomNumbers_parallel-tmpl.h:146-230,
+ *  randomNumbers_parallel_Mseries.cpp:29-38, randomNumbers_Mseries.cpp:29-145):
+ * an M-series master generator and one per (z, t) slice of the lattice.
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_rng_s* dwf_rng;
+/* RandomNumbers_parallel(seed) on a lattice of these dimensions (one process) */
+int dwf_rng_create(const int lattice_size[4], unsigned long seed, dwf_rng* out);
+/* reset(seed): re-seed every generator (init_2d) */
+int dwf_rng_reset(dwf_rng r, unsigned long seed);
+int dwf_rng_destroy(dwf_rng r);
+/* get(): one uniform number in [0, 1) from the master generator */
+int dwf_rng_get(dwf_rng r, double* value);
+/* gauss_lex_global(Field(nin, V, nex)): Gaussian numbers of variance 1/2 into
+ * the host array out[j][site][i] (lexical sites; nin even). */
+int dwf_rng_gauss_lex_global(dwf_rng r, int nin, int nex, double* out);
+
+/* ------------------------------------------------------------------------
+ * HMC gauge part, FP64, device arrays in the Field_G layout (4, V, 18).
+ * ---------------------------------------------------------------------- */
+/* Langevin_Momentum::set_iP (src/lib/HMC/langevin_Momentum.cpp:146-200): iP from
+ * 8 Gaussian numbers per link, host array gauss[mu][site][8] (as
+ * dwf_rng_gauss_lex_global(r, 8, 4, .) gives them); H_kin = |iP|^2 / 2. */
+int dwf_hmc_set_iP(const int lattice_size[4], double* iP, const double* gauss_host,
+                   double* H_kin);
+/* Integrator_UpdateU::evolve: U <- exp(alpha iP) U, nested Taylor form of order
+ * nprec and reunitarisation (mult_exp_Field_G, field_G_imp.cpp:496-520). */
+int dwf_hmc_exp_update(const int lattice_size[4], double* U, double alpha, const double* iP,
+                       int nprec);
+/* Action_G_Rectangle::calcH (src/lib/Action/Gauge/action_G_Rectangle.cpp:113-193):
+ * H = beta [c_plaq (6V - sum Re tr P / 3) + c_rect (12V - sum Re tr R / 3)];
+ * plaq = the average plaquette (may be NULL). */
+int dwf_hmc_rect_action(const int lattice_size[4], const double* U, double beta, double c_plaq,
+                        double c_rect, double* H, double* plaq);
+/* Force_G_Rectangle::force_core (src/lib/Force/Gauge/force_G_Rectangle.cpp:84-216):
+ * F_mu(x) = -(beta / 3) at(U_mu(x) S_mu(x)^+), S the staple sum (F overwritten). */
+int dwf_hmc_rect_force(const int lattice_size[4], double* F, const double* U, double beta,
+                       double c_plaq, double c_rect);
+/* y += a x, y = x, |x|^2 for device double arrays of n elements */
+int dwf_device_axpy(double* y, double a, const double* x, size_t n);
+int dwf_device_copy(double* y, const double* x, size_t n);
+int dwf_device_norm2(const double* x, size_t n, double* result);
+/* Average plaquette of a device Field_G. */
+int dwf_plaquette_device(const int lattice_size[4], const double* U, double* result);
+
 /* Average plaquette of a host Field_G (1 for the unit field). */
 int dwf_plaquette(const int lattice_size[4], const double* U_host, double* result);
 

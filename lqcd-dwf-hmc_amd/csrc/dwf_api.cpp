@@ -3017,6 +3017,98 @@ int dwf_fopr_set_config_smeared(dwf_fopr op, dwf_smear sm, const double* U_host)
   return DWF_OK;
 }
 
+// ---- HMC gauge part -----------------------------------------------------------
+static size_t lat_volume(const int dims[4]) {
+  return (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+}
+
+int dwf_hmc_set_iP(const int dims[4], double* iP, const double* gauss_host, double* H_kin) {
+  CTX();
+  if (!dims || !iP || !gauss_host || !H_kin) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  const size_t nl = 4 * lat_volume(dims);
+  double* h = nullptr;
+  HIPCHK(hipMalloc(&h, nl * 8 * sizeof(double)));
+  int rc = DWF_OK;
+  unsigned nb = 0;
+  if (hipMemcpyAsync(h, gauss_host, nl * 8 * sizeof(double), hipMemcpyHostToDevice,
+                     g_ctx.stream) != hipSuccess ||
+      launch_set_iP(iP, h, nl, g_ctx.red.partial, &nb, g_ctx.stream) != hipSuccess)
+    rc = fail(DWF_ERR_HIP, "set_iP kernel failed");
+  if (rc == DWF_OK) {
+    HIPCHK(launch_finish_partials(1, g_ctx.red.partial, nb, g_ctx.red.result, g_ctx.stream));
+    double n2;
+    rc = fetch(1, &n2);
+    if (rc == DWF_OK) *H_kin = 0.5 * n2;
+  }
+  (void)hipStreamSynchronize(g_ctx.stream);
+  (void)hipFree(h);
+  return rc;
+}
+
+int dwf_hmc_exp_update(const int dims[4], double* U, double alpha, const double* iP, int nprec) {
+  CTX();
+  if (!dims || !U || !iP || nprec < 1) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  HIPCHK(launch_exp_update(U, alpha, iP, U, 4 * lat_volume(dims), nprec, g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_hmc_rect_action(const int dims[4], const double* U, double beta, double c_plaq,
+                        double c_rect, double* H, double* plaq) {
+  CTX();
+  if (!dims || !U || !H) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  unsigned nb = 0;
+  HIPCHK(launch_rect_action(U, dims, g_ctx.red.partial, &nb, g_ctx.stream));
+  HIPCHK(launch_finish_partials(2, g_ctx.red.partial, nb, g_ctx.red.result, g_ctx.stream));
+  double v[2];
+  RC(fetch(2, v));
+  const double V = (double)lat_volume(dims);
+  const double Ndim2 = 6.0, Nc = 3.0;
+  // Action_G_Rectangle::calcH (action_G_Rectangle.cpp:183-186)
+  *H = beta * (c_plaq * Ndim2 * V - c_plaq * v[0] / Nc + c_rect * Ndim2 * V * 2 -
+               c_rect * v[1] / Nc);
+  if (plaq) *plaq = v[0] / Nc / V / Ndim2;
+  return DWF_OK;
+}
+
+int dwf_hmc_rect_force(const int dims[4], double* F, const double* U, double beta, double c_plaq,
+                       double c_rect) {
+  CTX();
+  if (!dims || !F || !U) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_rect_force(F, U, dims, beta, c_plaq, c_rect, g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_device_axpy(double* y, double a, const double* x, size_t n) {
+  CTX();
+  if (!y || !x) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_axpy(P64, y, a, x, n, g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_device_copy(double* y, const double* x, size_t n) {
+  CTX();
+  if (!y || !x) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, g_ctx.stream));
+  return DWF_OK;
+}
+
+int dwf_device_norm2(const double* x, size_t n, double* result) {
+  CTX();
+  if (!x || !result) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_norm2(P64, x, n, g_ctx.red, 0, g_ctx.stream));
+  return fetch(1, result);
+}
+
+int dwf_plaquette_device(const int dims[4], const double* U, double* result) {
+  CTX();
+  if (!dims || !U || !result) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  HIPCHK(launch_plaquette(U, dims[0], dims[1], dims[2], dims[3], g_ctx.red, 0, g_ctx.stream));
+  double sum = 0;
+  RC(fetch(1, &sum));
+  *result = sum / (3.0 * 6.0 * (double)lat_volume(dims));
+  return DWF_OK;
+}
+
 int dwf_plaquette(const int dims[4], const double* U_host, double* result) {
   CTX();
   if (!dims || !U_host || !result) return fail(DWF_ERR_INVALID_ARG, "null argument");

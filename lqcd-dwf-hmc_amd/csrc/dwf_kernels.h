@@ -119,6 +119,20 @@ struct MsLaunch {
 hipError_t launch_ms_blas(int prec, const MsLaunch& L, int stage, hipStream_t st);
 hipError_t launch_ms_ctrl(const MsLaunch& L, hipStream_t st);
 
+// result[0..nr) = sums of nblocks block partials of nr values each
+// (partial[block * nr + r]), in a fixed order; nr = 1 or 2
+hipError_t launch_finish_partials(int nr, const double* partial, unsigned nblocks,
+                                  double* result, hipStream_t st);
+// HMC gauge kernels (dwf_hmc.hip), FP64 Field_G arrays [mu][site][18]
+hipError_t launch_set_iP(double* P, const double* H, size_t nlinks, double* partial,
+                         unsigned* nblk, hipStream_t st);
+hipError_t launch_exp_update(double* W, double alpha, const double* P, const double* U,
+                             size_t nlinks, int nprec, hipStream_t st);
+hipError_t launch_rect_action(const double* U, const int dims[4], double* partial,
+                              unsigned* nblk, hipStream_t st);
+hipError_t launch_rect_force(double* F, const double* U, const int dims[4], double beta,
+                             double c_plaq, double c_rect, hipStream_t st);
+
 hipError_t prepare_dwf_group(int prec, int op, int Ls, int n, const GroupMember* m,
                              GroupLaunch* gl);
 hipError_t launch_dwf_group(const GroupLaunch& gl, hipStream_t st);

@@ -1963,6 +1963,19 @@ static void red_finish(const double* partial, unsigned nblocks, double* result, 
                      result, rstride, (double*)nullptr);
 }
 
+hipError_t launch_finish_partials(int nr, const double* partial, unsigned nblocks,
+                                  double* result, hipStream_t st) {
+  if (nr == 1)
+    hipLaunchKernelGGL(k_red_finish<1>, dim3(1, 1), dim3(256), 0, st, partial, nblocks, (size_t)0,
+                       result, 0, (double*)nullptr);
+  else if (nr == 2)
+    hipLaunchKernelGGL(k_red_finish<2>, dim3(1, 1), dim3(256), 0, st, partial, nblocks, (size_t)0,
+                       result, 0, (double*)nullptr);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 static void finish_norm(const double* partial, unsigned nblocks, double* result, hipStream_t st,
                         double* host) {
   hipLaunchKernelGGL(k_red_finish<1>, dim3(1, 1), dim3(256), 0, st, partial, nblocks, (size_t)0,

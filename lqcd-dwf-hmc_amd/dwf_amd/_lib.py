@@ -155,6 +155,19 @@ _SIGS = {
     "dwf_smear_mult_generator": (C.c_int, [_vp, _vp]),
     "dwf_fopr_set_config_smeared": (C.c_int, [_vp, _vp, _dp]),
     "dwf_plaquette": (C.c_int, [_ip, _dp, _dp]),
+    "dwf_plaquette_device": (C.c_int, [_ip, _vp, _dp]),
+    "dwf_rng_create": (C.c_int, [_ip, C.c_ulong, C.POINTER(_vp)]),
+    "dwf_rng_reset": (C.c_int, [_vp, C.c_ulong]),
+    "dwf_rng_destroy": (C.c_int, [_vp]),
+    "dwf_rng_get": (C.c_int, [_vp, _dp]),
+    "dwf_rng_gauss_lex_global": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "dwf_hmc_set_iP": (C.c_int, [_ip, _vp, _dp, _dp]),
+    "dwf_hmc_exp_update": (C.c_int, [_ip, _vp, C.c_double, _vp, C.c_int]),
+    "dwf_hmc_rect_action": (C.c_int, [_ip, _vp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+    "dwf_hmc_rect_force": (C.c_int, [_ip, _vp, _vp, C.c_double, C.c_double, C.c_double]),
+    "dwf_device_axpy": (C.c_int, [_vp, C.c_double, _vp, C.c_size_t]),
+    "dwf_device_copy": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "dwf_device_norm2": (C.c_int, [_vp, C.c_size_t, _dp]),
 }
 
 _lib = None

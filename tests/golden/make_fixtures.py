@@ -39,6 +39,31 @@ def main():
             "source": "sample_1gpu/run.log:1558 (plaq = 6.2512994697494251e-01)",
         },
     }
+    # energies printed by the 8^4 sample trajectory (sample_1gpu/run.log): the
+    # reference's HMC_DWF_Nf2p1_eo.yaml on conf_04040408.txt tiled to 8^4,
+    # RandomNumbers_parallel<Mseries> reset to 200000
+    log = open(os.path.join(REF, "sample_1gpu", "run.log")).read().split("\n")
+    want = [  # (key, line number, text before the number)
+        ("H_kin_init", 1512, "H_kin        ="),
+        ("H_Frational_init", 1523, "H_Frational  ="),
+        ("H_Fratio_Nf2_init", 1529, "H_Fratio     ="),
+        ("H_Fratio_Nf2prec_init", 1542, "H_Fratio     ="),
+        ("H_Grectangle_init", 1554, "H_Grectangle ="),
+        ("H_total_init", 1557, "H_total(init)  ="),
+        ("H_kin_trial", 1957, "H_kin        ="),
+        ("H_Frational_trial", 1968, "H_Frational  ="),
+        ("H_Fratio_Nf2_trial", 1975, "H_Fratio     ="),
+        ("H_Fratio_Nf2prec_trial", 1982, "H_Fratio     ="),
+        ("H_Grectangle_trial", 1987, "H_Grectangle ="),
+        ("H_total_trial", 1990, "H_total(trial) ="),
+        ("plaq_trial", 1992, "plaq(trial)    ="),
+        ("H_diff", 1994, "H(diff)       ="),
+    ]
+    for key, line, text in want:
+        row = log[line - 1]
+        assert text in row, (key, row)
+        known[key] = {"value": float(row.split("=")[1]),
+                      "source": "sample_1gpu/run.log:%d (%s)" % (line, row.strip())}
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump(known, f, indent=1)
 

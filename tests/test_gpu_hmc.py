@@ -1,0 +1,110 @@
+"""HMC on the GPU: the gauge kernels against the oracle, and the reference's
+own 8^4 trajectory against the energies it printed.
+
+The trajectory is benchmark_hmc_dwf with HMC_DWF_Nf2p1_eo.yaml (2+1 flavours:
+rational ratio PV/Nf1, Hasenbusch ratios Nf2_prec/Nf2 and PV/Nf2_prec, Ls = 12,
+b = 1.5, c = 0.5, M0 = 1, 3 stout levels, rectangle gauge action, Omelyan
+[2, 1, 3], tau = 0.1) from conf_04040408.txt tiled to 8^4 with
+RandomNumbers_parallel<Mseries> reset to 200000: sample_1gpu/run.log:1512-1994.
+Bars: the Langevin energies to 1e-10 relative (they are sums of squares of the
+reference's random numbers), the trial energies to 1e-6 absolute and
+H(diff) = 0.00083995 to 1e-6 absolute (the reference prints 8 decimals).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gauge_oracle as go
+import dwf_amd
+from dwf_amd import hmc, lattice
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KNOWN = json.load(open(os.path.join(HERE, "golden", "known_answers.json")))
+RATIONAL = os.path.join(HERE, "golden", "rational")
+DIMS8 = (8, 8, 8, 8)
+BETA, CP, CR = 4.17, 1.66666666666666666666, -0.08333333333333333333
+
+
+def conf8():
+    return lattice.tile_4x4x4x8(np.load(os.path.join(HERE, "golden", "conf_04040408.npy")), DIMS8)
+
+
+def dev(U, dims):
+    g = hmc.GaugeField(dims)
+    g.from_host(np.ascontiguousarray(U))
+    return g
+
+
+def test_rect_action_and_force_match_oracle():
+    for dims, U in ((DIMS8, conf8()), ((4, 4, 4, 6), lattice.random_gauge((4, 4, 4, 6), 0.6, 3))):
+        a = hmc.Action_G_Rectangle(dims, BETA, CP, CR)
+        a.set_config(dev(U, dims))
+        H = a.calcH()
+        Ho, plo = go.rect_action(U, dims, BETA, CP, CR)
+        assert abs(H - Ho) / Ho < 1e-13 and abs(a.plaquette - plo) < 1e-14
+        F = hmc.GaugeField(dims)
+        a.force(F)
+        Fo = go.rect_force(U, dims, BETA, CP, CR)
+        assert np.abs(F.to_host() - Fo).max() / np.abs(Fo).max() < 1e-13
+
+
+def test_rect_action_reference_value():
+    a = hmc.Action_G_Rectangle(DIMS8, BETA, CP, CR)
+    a.set_config(dev(conf8(), DIMS8))
+    assert abs(a.calcH() - KNOWN["H_Grectangle_init"]["value"]) < 1e-7
+
+
+def test_momenta_and_exp_update_match_oracle():
+    r = hmc.RandomNumbers_parallel_Mseries(DIMS8, 1234567)
+    r.reset(200000)
+    g = r.gauss_lex_global(8, 4)
+    P = hmc.GaugeField(DIMS8)
+    Hk = np.zeros(1)
+    dwf_amd._lib.check("dwf_hmc_set_iP", dwf_amd._lib.load().dwf_hmc_set_iP(
+        hmc._dims(DIMS8), P.ptr, hmc._ptr(np.ascontiguousarray(g)), hmc._ptr(Hk)))
+    Po, Hko = go.set_iP(g)
+    assert np.abs(P.to_host() - Po).max() < 1e-15
+    assert abs(Hk[0] - KNOWN["H_kin_init"]["value"]) < 1e-7
+    U = dev(conf8(), DIMS8)
+    upd = hmc.Integrator_UpdateU([], 8)
+    upd.evolve(0.0123, P, U)
+    Uo = go.exp_update(conf8(), 0.0123, Po, 8)
+    assert np.abs(U.to_host() - Uo).max() < 1e-14
+
+
+@pytest.fixture(scope="module")
+def trajectory():
+    hmcobj, rng = hmc.benchmark_nf2p1(DIMS8, RATIONAL)
+    rng.reset(200000)
+    U = dev(conf8(), DIMS8)
+    print("case: reference 8^4 trajectory (HMC_DWF_Nf2p1_eo.yaml)", flush=True)
+    out = hmcobj.update(U)
+    print("  init", out["init"], "\n  trial", out["trial"], "\n  H_diff", out["H_diff"],
+          "plaq", out["plaq_trial"], flush=True)
+    return out
+
+
+def test_trajectory_langevin_energies(trajectory):
+    h = trajectory["init"]
+    pairs = [("H_kin", "H_kin_init"), ("H_0_Frational", "H_Frational_init"),
+             ("H_1_Fratio_Nf2", "H_Fratio_Nf2_init"), ("H_2_Fratio_Nf2prec", "H_Fratio_Nf2prec_init"),
+             ("H_3_gauge", "H_Grectangle_init")]
+    for mine, ref in pairs:
+        want = KNOWN[ref]["value"]
+        assert abs(h[mine] - want) / want < 1e-10, (mine, h[mine], want)
+    assert abs(trajectory["plaq_init"] - KNOWN["plaq_init_8x8x8x8_sample_1gpu"]["value"]) < 1e-13
+
+
+def test_trajectory_trial_energies_and_delta_H(trajectory):
+    h = trajectory["trial"]
+    pairs = [("H_kin", "H_kin_trial"), ("H_0_Frational", "H_Frational_trial"),
+             ("H_1_Fratio_Nf2", "H_Fratio_Nf2_trial"), ("H_2_Fratio_Nf2prec", "H_Fratio_Nf2prec_trial"),
+             ("H_3_gauge", "H_Grectangle_trial")]
+    for mine, ref in pairs:
+        assert abs(h[mine] - KNOWN[ref]["value"]) < 1e-6, (mine, h[mine], KNOWN[ref]["value"])
+    assert abs(trajectory["plaq_trial"] - KNOWN["plaq_trial"]["value"]) < 1e-11
+    assert abs(trajectory["H_diff"] - KNOWN["H_diff"]["value"]) < 1e-6
