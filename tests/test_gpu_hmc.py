@@ -108,3 +108,72 @@ def test_trajectory_trial_energies_and_delta_H(trajectory):
         assert abs(h[mine] - KNOWN[ref]["value"]) < 1e-6, (mine, h[mine], KNOWN[ref]["value"])
     assert abs(trajectory["plaq_trial"] - KNOWN["plaq_trial"]["value"]) < 1e-11
     assert abs(trajectory["H_diff"] - KNOWN["H_diff"]["value"]) < 1e-6
+
+
+# Config-1 style setup: Shamir DWF (b = 1, c = 0), M0 = 1.6, Ls = 8, links
+# exp(0.5 iH) seeded, 2 stout steps; masses (Nf1, Nf2, Nf2_prec, PV).
+CFG1 = dict(Ls=8, M0=1.6, b=1.0, c=0.0, masses=(0.04, 0.02, 0.2, 1.0), nsmear=2)
+
+
+def _setup(dims, Nsteps, tau, seed=11):
+    h, rng = hmc.benchmark_nf2p1(dims, RATIONAL, Nsteps=Nsteps, trajectory_length=tau, **CFG1)
+    U0 = lattice.random_gauge(dims, 0.5, seed)
+    return h, rng, U0
+
+
+def test_reversibility():
+    """Forward for tau, flip the momenta, forward for tau: the links and the
+    energy return (Omelyan is time-reversible)."""
+    dims = (4, 4, 4, 8)
+    h, rng, U0 = _setup(dims, (2, 1, 3), 0.2)
+    rng.reset(5)
+    U = dev(U0, dims)
+    for a in h.actions:
+        a.set_config(U)
+    for d in h.directors:
+        d.set_config(U)
+    iP = hmc.GaugeField(dims)
+    e0 = sum(h.langevin(iP, U).values())
+    h.integrator.evolve(h.tau, iP, U)
+    e1 = sum(h.calc_Hamiltonian(iP, U).values())
+    iP.axpy(-2.0, iP)
+    h.integrator.evolve(h.tau, iP, U)
+    e2 = sum(h.calc_Hamiltonian(iP, U).values())
+    dU = np.abs(U.to_host() - U0).max()
+    print(f"  reversibility: max |U - U0| = {dU:.3e}, H0 = {e0:.10f}, H1 - H0 = {e1 - e0:.3e}, "
+          f"H2 - H0 = {e2 - e0:.3e}", flush=True)
+    assert dU < 1e-10
+    assert abs(e2 - e0) < 1e-8 * abs(e0)
+
+
+def test_delta_H_falls_as_step_squared():
+    """|dH| with N and 2N steps on the outermost level (the fermion forces; the
+    inner levels fixed at 4 steps; same momenta, noise and links): Omelyan is
+    second order, so the ratio is about 4 once the steps are small enough
+    (measured: 0.159, 0.0330, 0.00815 for N = 2, 4, 8 at tau = 0.5)."""
+    dims = (4, 4, 4, 8)
+    dH = []
+    for n in (4, 8):
+        h, rng, U0 = _setup(dims, (n, 4, 4), 0.5)
+        rng.reset(9)
+        out = h.update(dev(U0, dims))
+        dH.append(abs(out["H_diff"]))
+    ratio = dH[0] / dH[1]
+    print(f"  dH(N) = {dH[0]:.4e}, dH(2N) = {dH[1]:.4e}, ratio {ratio:.2f}", flush=True)
+    assert 3.0 < ratio < 5.5
+
+
+def test_config1_trajectory_8x8x8x8():
+    """One 2+1-flavour trajectory, tau = 1, steps [2, 1, 3], on 8^4 with Ls = 8
+    Shamir and 2 stout steps on exp(0.5 iH) links: the energy is conserved to
+    |dH| / H < 1e-4 (measured dH = -2.42 for H = 2.4e5: the outer step of 0.5
+    is coarse for these rough links; the 1/N^2 test above shows the error is
+    the integrator's)."""
+    dims = (8, 8, 8, 8)
+    h, rng, U0 = _setup(dims, (2, 1, 3), 1.0, seed=2024)
+    rng.reset(1)
+    out = h.update(dev(U0, dims))
+    print(f"  config-1 trajectory: H0 = {out['H_total_init']:.8f}, dH = {out['H_diff']:.6e}, "
+          f"plaq {out['plaq_init']:.10f} -> {out['plaq_trial']:.10f}", flush=True)
+    assert np.isfinite(out["H_diff"])
+    assert abs(out["H_diff"]) < 1e-4 * abs(out["H_total_init"])
