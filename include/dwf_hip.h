@@ -416,6 +416,14 @@ int dwf_device_norm2(const double* x, size_t n, double* result);
 /* Average plaquette of a device Field_G. */
 int dwf_plaquette_device(const int lattice_size[4], const double* U, double* result);
 
+/* Gauge configuration files, "Binary" (ILDG order, big-endian; GaugeConfig
+ * ("Binary"), src/lib/IO/gaugeConfig.cpp:61-107, fieldIO_Binary.cpp:29-176,
+ * io_format_gauge.h:43-70): per lexical site the 4 x 18 doubles of its links.
+ * U_host is a Field_G (4, V, 18).  DWF_ERR_INVALID_ARG if the file cannot be
+ * opened, is short, or cannot be written.  No device needed. */
+int dwf_gauge_read_binary(const char* path, const int lattice_size[4], double* U_host);
+int dwf_gauge_write_binary(const char* path, const int lattice_size[4], const double* U_host);
+
 /* Average plaquette of a host Field_G (1 for the unit field). */
 int dwf_plaquette(const int lattice_size[4], const double* U_host, double* result);
 

@@ -156,6 +156,8 @@ _SIGS = {
     "dwf_fopr_set_config_smeared": (C.c_int, [_vp, _vp, _dp]),
     "dwf_plaquette": (C.c_int, [_ip, _dp, _dp]),
     "dwf_plaquette_device": (C.c_int, [_ip, _vp, _dp]),
+    "dwf_gauge_read_binary": (C.c_int, [C.c_char_p, _ip, _dp]),
+    "dwf_gauge_write_binary": (C.c_int, [C.c_char_p, _ip, _dp]),
     "dwf_rng_create": (C.c_int, [_ip, C.c_ulong, C.POINTER(_vp)]),
     "dwf_rng_reset": (C.c_int, [_vp, C.c_ulong]),
     "dwf_rng_destroy": (C.c_int, [_vp]),

@@ -121,6 +121,33 @@ def tile_4x4x4x8(file_order, dims):
     return np.ascontiguousarray(np.transpose(file_order[g], (1, 0, 2)))
 
 
+def read_binary(path, dims):
+    """GaugeConfig("Binary"): ILDG order, big-endian (libdwfhip's reader).
+    Returns a Field_G float64 (4, V, 18) array."""
+    import ctypes as C
+    from . import _lib
+    U = np.empty((4, volume(dims), 18))
+    d = (C.c_int * 4)(*[int(v) for v in dims])
+    rc = _lib.load().dwf_gauge_read_binary(os.fsencode(path), d,
+                                            U.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise IOError(f"{path}: not a Binary gauge file of lattice {tuple(dims)}")
+    return U
+
+
+def write_binary(path, dims, U):
+    """GaugeConfig("Binary").write_file: ILDG order, big-endian."""
+    import ctypes as C
+    from . import _lib
+    U = np.ascontiguousarray(U, np.float64)
+    assert U.shape == (4, volume(dims), 18)
+    d = (C.c_int * 4)(*[int(v) for v in dims])
+    rc = _lib.load().dwf_gauge_write_binary(os.fsencode(path), d,
+                                             U.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise IOError(f"{path}: write failed")
+
+
 # ---------------------------------------------------------------------------
 # rational approximation files (src/lib/Tools/math_Rational.cpp, read_file)
 # ---------------------------------------------------------------------------
