@@ -262,6 +262,44 @@ int dwf_solver_create_richardson_lpc(dwf_fopr op_d, dwf_fopr op_f, dwf_lpc prec,
                                      double stop_cond_inner, dwf_solver* out);
 
 /* ------------------------------------------------------------------------
+ * Learned preconditioner: a gauge-covariant parallel-transport convolution
+ * network (no counterpart in the reference; a new solver component for the
+ * same operator).  Layer l maps channels[l] to channels[l + 1] 5D spinor
+ * fields on the full lattice:
+ *   out[a](x, s) = sum_{q < 11} sum_b W_l[q][a][b] T_q(in[b])(x, s),
+ * T_0 = 1, T_{1+dir} = the 8 gauge hops of the operator (U(x, dir) in(x + dir),
+ * boundary phases included), T_9 / T_10 = the shifts s -> s + 1 / s - 1 (zero
+ * beyond the walls); W_l[q][a][b] a complex 4 x 4 spin matrix, stored as
+ * bfloat16 (round to nearest even), applied in FP32.  The input is an even
+ * field (odd parity zero), the output the even part of the last layer.
+ * channels[0] = channels[nlayers] = 1, others <= 16.  op: an FP32 operator
+ * whose links (after set_config) the network transports with.
+ * Weights (host float / double): per layer [q][c_out][c_in][16 x (re, im)],
+ * row-major spin entries, layers one after another (dwf_ptc_num_weights).
+ * ---------------------------------------------------------------------- */
+typedef struct dwf_ptc_s* dwf_ptc;
+int dwf_ptc_create(dwf_fopr op, int nlayers, const int* channels, dwf_ptc* out);
+int dwf_ptc_destroy(dwf_ptc n);
+size_t dwf_ptc_num_weights(dwf_ptc n);
+/* W rounded to bfloat16; get returns the rounded values */
+int dwf_ptc_set_weights(dwf_ptc n, const float* W);
+int dwf_ptc_get_weights(dwf_ptc n, float* W);
+/* out = N(in), FP32 fields of op */
+int dwf_ptc_apply(dwf_ptc n, void* out, const void* in);
+/* loss = sum_k |N(b[k]) - x[k]|^2 over nbatch pairs of FP32 fields of op and
+ * grad = d loss / d W (host array of dwf_ptc_num_weights doubles, d/d re and
+ * d/d im of every entry), by the network's forward and adjoint kernels. */
+int dwf_ptc_loss_grad(dwf_ptc n, int nbatch, const void* const* b, const void* const* x,
+                      double* loss, double* grad);
+/* Flexible CG on op (mode DdagD) preconditioned by the network, and the FP64
+ * Richardson around it (the inner flexible CG on op_f), as for the polynomial. */
+int dwf_solver_create_fcg_ptc(dwf_fopr op, dwf_ptc prec, int max_iter, double stop_cond,
+                              dwf_solver* out);
+int dwf_solver_create_richardson_ptc(dwf_fopr op_d, dwf_fopr op_f, dwf_ptc prec, int max_iter,
+                                     double stop_cond, int max_iter_inner,
+                                     double stop_cond_inner, dwf_solver* out);
+
+/* ------------------------------------------------------------------------
  * DWF fermion force, FP64: AForce_F_Domainwall_5din_eo
  * (src/lib_alt_QXS/Force/Fermion/aforce_F_Domainwall_5din_eo-tmpl.h:299-451,
  *  lexical part aforce_F_Domainwall_5din-tmpl.h:338-933, tensorProd_5din

@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "dwf_amd", "libdwfhip.so")
 OBJ = os.path.join(HERE, "_build")
 
 SOURCES = ["dwf_kernels.hip", "dwf_smear.hip", "dwf_force.hip", "dwf_lpc.hip", "dwf_batch.hip",
-           "dwf_hmc.hip", "dwf_rng.cpp", "dwf_io.cpp", "dwf_api.cpp"]
+           "dwf_hmc.hip", "dwf_ptc.hip", "dwf_rng.cpp", "dwf_io.cpp", "dwf_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

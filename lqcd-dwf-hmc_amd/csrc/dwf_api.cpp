@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1159,8 +1160,11 @@ struct dwf_solver_s {
   dwf_solver_s* inner = nullptr;
   int inner_iters = 0;
   dwf_lpc_s* lpc = nullptr;
+  struct dwf_ptc_s* ptc = nullptr;   // flexible CG: the network preconditioner instead
   bool fast = false;   // inner CG: the sync-free iteration (cg_solve_fast)
 };
+static int ptc_apply_impl(struct dwf_ptc_s* n, void* out, const void* in);
+static dwf_fopr ptc_op(struct dwf_ptc_s* n);
 
 static int solver_field(dwf_fopr op, void** slot) {
   if (!*slot) HIPCHK(hipMalloc(slot, op->nelem * prec_bytes(op->prec)));
@@ -1404,6 +1408,17 @@ static int cg_solve_fast(dwf_solver_s* s, void* xq, const void* b, int* nconv_ou
 // The preconditioner may run in another precision than op (conversions at
 // its input and output).
 static int lpc_apply_conv(dwf_solver_s* s, void* z, const void* r) {
+  if (s->ptc) {   // the network runs on FP32 fields of its own operator
+    dwf_fopr po = ptc_op(s->ptc);
+    dwf_fopr op = s->op;
+    if (po->prec == op->prec) return ptc_apply_impl(s->ptc, z, r);
+    RC(solver_field(po, &s->g[0]));
+    RC(solver_field(po, &s->g[1]));
+    HIPCHK(launch_convert(po->prec, s->g[0], op->prec, r, op->nelem, g_ctx.stream));
+    RC(ptc_apply_impl(s->ptc, s->g[1], s->g[0]));
+    HIPCHK(launch_convert(op->prec, z, po->prec, s->g[1], op->nelem, g_ctx.stream));
+    return DWF_OK;
+  }
   dwf_lpc_s* P = s->lpc;
   dwf_fopr op = s->op;
   if (P->op->prec == op->prec) return lpc_apply_M(P, z, r);
@@ -3016,6 +3031,232 @@ int dwf_fopr_set_config_smeared(dwf_fopr op, dwf_smear sm, const double* U_host)
   HIPCHK(hipStreamSynchronize(g_ctx.stream));
   return DWF_OK;
 }
+
+// ---- parallel-transport convolution network (learned preconditioner) ----------
+// Layers l = 0 .. L-1 map ch[l] to ch[l + 1] channels; the input is the even
+// field (channel 0 of the even parity, odd parity zero), the output the even
+// part of channel 0 of the last layer.  Weights: per layer [q][c_out][c_in]
+// x 16 complex spin entries (re, im), bfloat16 on the device.
+struct dwf_ptc_s {
+  dwf_fopr op = nullptr;                 // FP32 operator: links, neighbours, Ls, Vh
+  std::vector<int> ch;                   // channels, L + 1 entries
+  std::vector<size_t> woff;              // float offset of each layer's weights
+  size_t nw = 0;
+  unsigned short* W = nullptr;           // device bf16
+  std::vector<float*> act;               // L + 1 activations (2 parities each)
+  float* g[2] = {nullptr, nullptr};      // backward ping-pong
+  double* partial = nullptr;             // weight-gradient block partials
+  double* G = nullptr;                   // one layer's gradient (device)
+  int nblk = 0;
+};
+
+static dwf_fopr ptc_op(dwf_ptc_s* n) { return n->op; }
+
+static unsigned short f32_to_bf16(float f) {   // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t r = ((u >> 16) & 1u) + 0x7fffu;
+  return (unsigned short)((u + r) >> 16);
+}
+static float bf16_to_f32(unsigned short h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+static int ptc_forward(dwf_ptc_s* n, const void* in) {
+  dwf_fopr op = n->op;
+  hipStream_t st = g_ctx.stream;
+  HIPCHK(launch_ptc_embed(n->act[0], (const float*)in, op->nelem, st));
+  for (size_t l = 0; l + 1 < n->ch.size(); ++l)
+    HIPCHK(launch_ptc_layer(false, n->act[l + 1], n->act[l], n->ch[l], n->ch[l + 1],
+                            n->W + n->woff[l], n->ch[l], n->ch[l + 1], (const float*)op->links,
+                            op->nbr, op->Vh, op->Ls, st));
+  return DWF_OK;
+}
+
+static int ptc_apply_impl(dwf_ptc_s* n, void* out, const void* in) {
+  RC(ptc_forward(n, in));
+  HIPCHK(hipMemcpyAsync(out, n->act.back(), n->op->nelem * sizeof(float),
+                        hipMemcpyDeviceToDevice, g_ctx.stream));
+  return DWF_OK;
+}
+
+extern "C" {
+
+int dwf_ptc_destroy(dwf_ptc n) {
+  if (!n) return DWF_OK;
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+  if (n->W) (void)hipFree(n->W);
+  for (auto* a : n->act)
+    if (a) (void)hipFree(a);
+  for (auto* a : n->g)
+    if (a) (void)hipFree(a);
+  if (n->partial) (void)hipFree(n->partial);
+  if (n->G) (void)hipFree(n->G);
+  delete n;
+  return DWF_OK;
+}
+
+int dwf_ptc_create(dwf_fopr op, int nlayers, const int* channels, dwf_ptc* out) {
+  CTX();
+  if (!op || !channels || !out || nlayers < 1 || nlayers > 8)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (op->prec != P32) return fail(DWF_ERR_UNSUPPORTED, "the network runs on an FP32 operator");
+  if (channels[0] != 1 || channels[nlayers] != 1)
+    return fail(DWF_ERR_INVALID_ARG, "first and last channel counts must be 1");
+  auto* n = new dwf_ptc_s;
+  n->op = op;
+  int cmax = 1;
+  for (int l = 0; l <= nlayers; ++l) {
+    if (channels[l] < 1 || channels[l] > 16) {
+      delete n;
+      return fail(DWF_ERR_INVALID_ARG, "1 <= channels <= 16");
+    }
+    n->ch.push_back(channels[l]);
+    cmax = std::max(cmax, channels[l]);
+  }
+  for (int l = 0; l < nlayers; ++l) {
+    n->woff.push_back(n->nw);
+    n->nw += (size_t)11 * channels[l] * channels[l + 1] * 32;
+  }
+  const size_t fb = op->nelem * sizeof(float);
+  auto bad = [&](const char* m) {
+    dwf_ptc_destroy(n);
+    return fail(DWF_ERR_HIP, m);
+  };
+  if (hipMalloc(&n->W, n->nw * sizeof(unsigned short)) != hipSuccess ||
+      hipMemset(n->W, 0, n->nw * sizeof(unsigned short)) != hipSuccess)
+    return bad("device allocation failed");
+  for (int l = 0; l <= nlayers; ++l) {
+    float* a = nullptr;
+    if (hipMalloc(&a, 2 * fb * channels[l]) != hipSuccess) return bad("device allocation failed");
+    n->act.push_back(a);
+  }
+  for (auto& g : n->g)
+    if (hipMalloc(&g, 2 * fb * cmax) != hipSuccess) return bad("device allocation failed");
+  n->nblk = 64;
+  if (hipMalloc(&n->partial, sizeof(double) * 32 * 11 * cmax * cmax * n->nblk) != hipSuccess ||
+      hipMalloc(&n->G, sizeof(double) * 32 * 11 * cmax * cmax) != hipSuccess)
+    return bad("device allocation failed");
+  *out = n;
+  return DWF_OK;
+}
+
+size_t dwf_ptc_num_weights(dwf_ptc n) { return n ? n->nw : 0; }
+
+int dwf_ptc_set_weights(dwf_ptc n, const float* W) {
+  CTX();
+  if (!n || !W) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  std::vector<unsigned short> h(n->nw);
+  for (size_t i = 0; i < n->nw; ++i) h[i] = f32_to_bf16(W[i]);
+  HIPCHK(hipMemcpy(n->W, h.data(), n->nw * sizeof(unsigned short), hipMemcpyHostToDevice));
+  return DWF_OK;
+}
+
+int dwf_ptc_get_weights(dwf_ptc n, float* W) {
+  CTX();
+  if (!n || !W) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  std::vector<unsigned short> h(n->nw);
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  HIPCHK(hipMemcpy(h.data(), n->W, n->nw * sizeof(unsigned short), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n->nw; ++i) W[i] = bf16_to_f32(h[i]);
+  return DWF_OK;
+}
+
+int dwf_ptc_apply(dwf_ptc n, void* out, const void* in) {
+  CTX();
+  if (!n || !out || !in) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (!n->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  return ptc_apply_impl(n, out, in);
+}
+
+int dwf_ptc_loss_grad(dwf_ptc n, int nbatch, const void* const* b, const void* const* x,
+                      double* loss, double* grad) {
+  CTX();
+  if (!n || nbatch < 1 || !b || !x || !loss || !grad)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (!n->op->have_config) return fail(DWF_ERR_NO_CONFIG, "set_config has not been called");
+  dwf_fopr op = n->op;
+  hipStream_t st = g_ctx.stream;
+  const int L = (int)n->ch.size() - 1;
+  std::fill(grad, grad + n->nw, 0.0);
+  *loss = 0;
+  std::vector<double> hG;
+  for (int k = 0; k < nbatch; ++k) {
+    RC(ptc_forward(n, b[k]));
+    unsigned nb = 0;
+    HIPCHK(launch_ptc_residual(n->g[0], n->act[L], (const float*)x[k], op->nelem,
+                               g_ctx.red.partial, &nb, st));
+    HIPCHK(launch_finish_partials(1, g_ctx.red.partial, nb, g_ctx.red.result, st));
+    double l2;
+    RC(fetch(1, &l2));
+    *loss += l2;
+    int cur = 0;
+    for (int l = L - 1; l >= 0; --l) {
+      const int cin = n->ch[l], cout = n->ch[l + 1];
+      // block count: about 1024 lattice points per block, at most nblk
+      const size_t pts = (size_t)2 * op->Vh * op->Ls;
+      const int nbk = (int)std::max<size_t>(1, std::min<size_t>(n->nblk, pts / 1024));
+      HIPCHK(launch_ptc_wgrad(n->G, n->partial, nbk, n->g[cur], n->act[l], cin, cout,
+                              (const float*)op->links, op->nbr, op->Vh, op->Ls, st));
+      const size_t ng = (size_t)11 * cin * cout * 32;
+      hG.resize(ng);
+      HIPCHK(hipMemcpyAsync(hG.data(), n->G, ng * sizeof(double), hipMemcpyDeviceToHost, st));
+      if (l > 0)
+        HIPCHK(launch_ptc_layer(true, n->g[1 - cur], n->g[cur], cout, cin, n->W + n->woff[l], cin,
+                                cout, (const float*)op->links, op->nbr, op->Vh, op->Ls, st));
+      HIPCHK(hipStreamSynchronize(st));
+      // [q][b][a][32] (block y = q cin + b, z = a) -> [q][a][b][32]; with
+      // g = dL/dRe out + i dL/dIm out, dL/dRe W + i dL/dIm W = G = sum g conj(T in)
+      double* gl = grad + n->woff[l];
+      for (int q = 0; q < 11; ++q)
+        for (int bb = 0; bb < cin; ++bb)
+          for (int a = 0; a < cout; ++a)
+            for (int e = 0; e < 16; ++e) {
+              const double* src = &hG[(((size_t)q * cin + bb) * cout + a) * 32];
+              double* dst = gl + (((size_t)q * cout + a) * cin + bb) * 32;
+              dst[2 * e] += src[2 * e];
+              dst[2 * e + 1] += src[2 * e + 1];
+            }
+      cur = 1 - cur;
+    }
+  }
+  return DWF_OK;
+}
+
+int dwf_solver_create_fcg_ptc(dwf_fopr op, dwf_ptc prec, int max_iter, double stop_cond,
+                              dwf_solver* out) {
+  CTX();
+  if (!op || !prec || !out || max_iter < 0) return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  if (op->nelem != prec->op->nelem) return fail(DWF_ERR_INVALID_ARG, "operators differ in size");
+  dwf_solver_s* s = new dwf_solver_s;
+  s->kind = 2;
+  s->op = op;
+  s->niter = max_iter;
+  s->stop = stop_cond;
+  s->guess = DWF_GUESS_ZERO;
+  s->ptc = prec;
+  *out = s;
+  return DWF_OK;
+}
+
+int dwf_solver_create_richardson_ptc(dwf_fopr op_d, dwf_fopr op_f, dwf_ptc prec, int max_iter,
+                                     double stop_cond, int max_iter_inner,
+                                     double stop_cond_inner, dwf_solver* out) {
+  CTX();
+  if (!op_d || !op_f || !prec || !out) return fail(DWF_ERR_INVALID_ARG, "null argument");
+  if (op_d->nelem != op_f->nelem || op_f->nelem != prec->op->nelem)
+    return fail(DWF_ERR_INVALID_ARG, "operators differ in size");
+  RC(dwf_solver_create_richardson(op_d, op_f, max_iter, stop_cond, max_iter_inner,
+                                  stop_cond_inner, out));
+  (*out)->inner->kind = 2;
+  (*out)->inner->ptc = prec;
+  return DWF_OK;
+}
+
+}  // extern "C"
 
 // ---- HMC gauge part -----------------------------------------------------------
 static size_t lat_volume(const int dims[4]) {

@@ -123,6 +123,16 @@ hipError_t launch_ms_ctrl(const MsLaunch& L, hipStream_t st);
 // (partial[block * nr + r]), in a fixed order; nr = 1 or 2
 hipError_t launch_finish_partials(int nr, const double* partial, unsigned nblocks,
                                   double* result, hipStream_t st);
+// parallel-transport convolution network (dwf_ptc.hip), FP32 activations
+hipError_t launch_ptc_layer(bool adj, float* out, const float* in, int cin, int cout,
+                            const unsigned short* W, int wcin, int wcout, const float* links,
+                            const int* nbr, int Vh, int Ls, hipStream_t st);
+hipError_t launch_ptc_wgrad(double* G, double* partial, int nblk, const float* g, const float* in,
+                            int cin, int cout, const float* links, const int* nbr, int Vh, int Ls,
+                            hipStream_t st);
+hipError_t launch_ptc_embed(float* act, const float* x, size_t n, hipStream_t st);
+hipError_t launch_ptc_residual(float* g, const float* act, const float* t, size_t n,
+                               double* partial, unsigned* nblk, hipStream_t st);
 // HMC gauge kernels (dwf_hmc.hip), FP64 Field_G arrays [mu][site][18]
 hipError_t launch_set_iP(double* P, const double* H, size_t nlinks, double* partial,
                          unsigned* nblk, hipStream_t st);

@@ -1,0 +1,279 @@
+// dwf_ptc.hip -- gauge-covariant parallel-transport convolution (PTC) network
+// kernels: the learned preconditioner of the flexible CG.
+//
+// A layer maps C_in channels to C_out channels of 5D spinor fields on the full
+// lattice (both parities):
+//   out[a](x, s) = sum_{q < 11} sum_b W[q][a][b] T_q(in[b])(x, s)
+// with W[q][a][b] a complex 4x4 matrix on spin (identity on colour) and the
+// transports
+//   q = 0          identity
+//   q = 1 + dir    U(x, dir) in(x + dir), dir = 2 mu + fb the 8 hops of the
+//                  operator (the packed links of the even-odd operator, boundary
+//                  phases included: forward U_mu(x), backward U_mu(x - mu)^+)
+//   q = 9, 10      in(x, s + 1), in(x, s - 1)   (zero beyond the walls)
+// Every T_q commutes with gauge transformations of in and out, so the layer
+// is gauge covariant.  The adjoint layer (the backward pass for the input
+// gradient) is the same kernel with the reversed transport (dir -> dir ^ 1,
+// 9 <-> 10) and W^+.  Weights are stored in bfloat16 and applied in FP32.
+//
+// Activation layout: channel-major, then parity, then the device spinor
+// layout of the operator, element ((site * 24 + k) * Ls + s), k = ri + 2 (c + 3 d).
+#include "dwf_device.h"
+#include "dwf_kernels.h"
+
+namespace dwf {
+
+constexpr int kPtcPaths = 11;
+
+__device__ __forceinline__ float bf16_to_f(unsigned short h) {
+  return __uint_as_float((unsigned)h << 16);
+}
+
+__device__ __forceinline__ int ptc_rev(int q) {
+  if (q == 0) return 0;
+  if (q <= 8) return 1 + ((q - 1) ^ 1);
+  return q == 9 ? 10 : 9;
+}
+
+// t = T_q(in)(p, site, s) for one channel field `in` (both parities, Vh each);
+// returns false if the path leaves the fifth dimension (t = 0)
+__device__ __forceinline__ bool ptc_transport(float (&t)[24], const float* __restrict__ in,
+                                              const float* __restrict__ links,
+                                              const int* __restrict__ nbr, int q, int p,
+                                              int site, int s, int Vh, int Ls) {
+  const size_t fsz = (size_t)Vh * 24 * Ls;
+  if (q == 0 || q >= 9) {
+    const int ss = q == 0 ? s : (q == 9 ? s + 1 : s - 1);
+    if (ss < 0 || ss >= Ls) return false;
+    const float* src = in + (size_t)p * fsz + (size_t)site * 24 * Ls + ss;
+#pragma unroll
+    for (int k = 0; k < 24; ++k) t[k] = src[k * Ls];
+    return true;
+  }
+  const int dir = q - 1;
+  const int nb = nbr[((size_t)p * 8 + dir) * Vh + site];
+  const float* src = in + (size_t)(1 - p) * fsz + (size_t)nb * 24 * Ls + s;
+  float w[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) w[k] = src[k * Ls];
+  const float* L = links + ((size_t)p * 8 + dir) * 18 * Vh + site;
+  float u[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) u[e] = L[(size_t)e * Vh];
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float re = 0, im = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const float ur = u[2 * (b + 3 * a)], ui = u[2 * (b + 3 * a) + 1];
+        const float wr = w[2 * (b + 3 * d)], wi = w[2 * (b + 3 * d) + 1];
+        re += ur * wr - ui * wi;
+        im += ur * wi + ui * wr;
+      }
+      t[2 * (a + 3 * d)] = re;
+      t[2 * (a + 3 * d) + 1] = im;
+    }
+  return true;
+}
+
+// One layer (ADJ = false) or its adjoint (ADJ = true): thread per
+// (parity, site, output channel, s), s fastest.  W: bf16 [q][a][b][alpha][beta][ri]
+// of the forward layer (C_out x C_in channels); for ADJ the kernel's
+// input channels are the forward outputs and out[b] = sum W^+ T_rev(q)(in[a]).
+template <bool ADJ>
+__global__ void __launch_bounds__(256)
+k_ptc_layer(float* __restrict__ out, const float* __restrict__ in, int cin, int cout,
+            const unsigned short* __restrict__ W, int wcin, int wcout,
+            const float* __restrict__ links, const int* __restrict__ nbr, int Vh, int Ls) {
+  const size_t fsz = (size_t)Vh * 24 * Ls;
+  const size_t total = (size_t)2 * Vh * cout * Ls;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(idx % Ls);
+    size_t r = idx / Ls;
+    const int a = (int)(r % cout);
+    r /= cout;
+    const int site = (int)(r % Vh);
+    const int p = (int)(r / Vh);
+    float acc[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) acc[k] = 0;
+    for (int q = 0; q < kPtcPaths; ++q) {
+      const int qw = ADJ ? ptc_rev(q) : q;   // weight path
+      for (int b = 0; b < cin; ++b) {
+        float t[24];
+        if (!ptc_transport(t, in + (size_t)b * 2 * fsz, links, nbr, q, p, site, s, Vh, Ls))
+          continue;
+        // forward: W[qw][a][b];  adjoint: (W[qw][b][a])^+
+        const int wa = ADJ ? b : a, wb = ADJ ? a : b;
+        const unsigned short* w = W + (((size_t)qw * wcout + wa) * wcin + wb) * 32;
+#pragma unroll
+        for (int al = 0; al < 4; ++al)
+#pragma unroll
+          for (int be = 0; be < 4; ++be) {
+            // element (al, be) of the applied spin matrix
+            const int i = ADJ ? (be * 4 + al) : (al * 4 + be);
+            const float wr = bf16_to_f(w[2 * i]);
+            const float wi = ADJ ? -bf16_to_f(w[2 * i + 1]) : bf16_to_f(w[2 * i + 1]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const float tr = t[2 * (c + 3 * be)], ti = t[2 * (c + 3 * be) + 1];
+              acc[2 * (c + 3 * al)] += wr * tr - wi * ti;
+              acc[2 * (c + 3 * al) + 1] += wr * ti + wi * tr;
+            }
+          }
+      }
+    }
+    float* o = out + (size_t)a * 2 * fsz + (size_t)p * fsz + (size_t)site * 24 * Ls + s;
+#pragma unroll
+    for (int k = 0; k < 24; ++k) o[k * Ls] = acc[k];
+  }
+}
+
+// Weight gradient of one layer: block (x, y = q * cin + b, z = a) adds, over
+// its share of the lattice, G[q][a][b][al][be] = sum_{x, s, c}
+// g[a](al, c) conj(T_q(in[b])(be, c)); block partials [y][z][block][32].
+__global__ void __launch_bounds__(256)
+k_ptc_wgrad(double* __restrict__ partial, const float* __restrict__ g,
+            const float* __restrict__ in, int cin, int cout, const float* __restrict__ links,
+            const int* __restrict__ nbr, int Vh, int Ls) {
+  const int q = blockIdx.y / cin, b = blockIdx.y - (blockIdx.y / cin) * cin;
+  const int a = blockIdx.z;
+  const size_t fsz = (size_t)Vh * 24 * Ls;
+  const size_t total = (size_t)2 * Vh * Ls;
+  float acc[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) acc[k] = 0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(idx % Ls);
+    const size_t r = idx / Ls;
+    const int site = (int)(r % Vh);
+    const int p = (int)(r / Vh);
+    float t[24];
+    if (!ptc_transport(t, in + (size_t)b * 2 * fsz, links, nbr, q, p, site, s, Vh, Ls)) continue;
+    const float* gp = g + (size_t)a * 2 * fsz + (size_t)p * fsz + (size_t)site * 24 * Ls + s;
+    float gv[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) gv[k] = gp[k * Ls];
+#pragma unroll
+    for (int al = 0; al < 4; ++al)
+#pragma unroll
+      for (int be = 0; be < 4; ++be)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float gr = gv[2 * (c + 3 * al)], gi = gv[2 * (c + 3 * al) + 1];
+          const float tr = t[2 * (c + 3 * be)], ti = t[2 * (c + 3 * be) + 1];
+          acc[2 * (al * 4 + be)] += gr * tr + gi * ti;
+          acc[2 * (al * 4 + be) + 1] += gi * tr - gr * ti;
+        }
+  }
+  __shared__ double sh[4][32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) sh[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const double v = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] +
+                     sh[3][threadIdx.x];
+    partial[(((size_t)blockIdx.y * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x) * 32 +
+            threadIdx.x] = v;
+  }
+}
+
+// out[(y z) 32 + k] = sum over the nblk block partials, fixed order
+__global__ void __launch_bounds__(64)
+k_ptc_wgrad_finish(double* __restrict__ out, const double* __restrict__ partial, int nblk,
+                   int nyz) {
+  const int yz = blockIdx.x;
+  const int k = threadIdx.x;
+  if (yz >= nyz || k >= 32) return;
+  double v = 0;
+  for (int i = 0; i < nblk; ++i) v += partial[((size_t)yz * nblk + i) * 32 + k];
+  out[(size_t)yz * 32 + k] = v;
+}
+
+// even part of channel 0 <- x (FP32 eo field), odd part <- 0
+__global__ void __launch_bounds__(256)
+k_ptc_embed(float* __restrict__ act, const float* __restrict__ x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    act[i] = x[i];
+    act[n + i] = 0.f;
+  }
+}
+
+// g = 2 (y - t) on the even part of channel 0 (y = act), 0 on the odd part;
+// block partials of |y - t|^2
+__global__ void __launch_bounds__(256)
+k_ptc_residual(float* __restrict__ g, const float* __restrict__ act, const float* __restrict__ t,
+               size_t n, double* __restrict__ partial) {
+  double acc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float d = act[i] - t[i];
+    acc += (double)d * (double)d;
+    g[i] = 2.f * d;
+    g[n + i] = 0.f;
+  }
+  __shared__ double sh[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) sh[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+static unsigned ptc_grid(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+hipError_t launch_ptc_layer(bool adj, float* out, const float* in, int cin, int cout,
+                            const unsigned short* W, int wcin, int wcout, const float* links,
+                            const int* nbr, int Vh, int Ls, hipStream_t st) {
+  const size_t total = (size_t)2 * Vh * cout * Ls;
+  if (adj)
+    hipLaunchKernelGGL(k_ptc_layer<true>, dim3(ptc_grid(total)), dim3(256), 0, st, out, in, cin,
+                       cout, W, wcin, wcout, links, nbr, Vh, Ls);
+  else
+    hipLaunchKernelGGL(k_ptc_layer<false>, dim3(ptc_grid(total)), dim3(256), 0, st, out, in, cin,
+                       cout, W, wcin, wcout, links, nbr, Vh, Ls);
+  return hipGetLastError();
+}
+
+hipError_t launch_ptc_wgrad(double* G, double* partial, int nblk, const float* g, const float* in,
+                            int cin, int cout, const float* links, const int* nbr, int Vh, int Ls,
+                            hipStream_t st) {
+  const dim3 grid(nblk, kPtcPaths * cin, cout);
+  hipLaunchKernelGGL(k_ptc_wgrad, grid, dim3(256), 0, st, partial, g, in, cin, cout, links, nbr,
+                     Vh, Ls);
+  const int nyz = kPtcPaths * cin * cout;
+  hipLaunchKernelGGL(k_ptc_wgrad_finish, dim3(nyz), dim3(64), 0, st, G, partial, nblk, nyz);
+  return hipGetLastError();
+}
+
+hipError_t launch_ptc_embed(float* act, const float* x, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_ptc_embed, dim3(ptc_grid(n)), dim3(256), 0, st, act, x, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_ptc_residual(float* g, const float* act, const float* t, size_t n,
+                               double* partial, unsigned* nblk, hipStream_t st) {
+  unsigned gr = ptc_grid(n);
+  if (gr > (unsigned)kRedBlocks) gr = kRedBlocks;
+  *nblk = gr;
+  hipLaunchKernelGGL(k_ptc_residual, dim3(gr), dim3(256), 0, st, g, act, t, n, partial);
+  return hipGetLastError();
+}
+
+}  // namespace dwf

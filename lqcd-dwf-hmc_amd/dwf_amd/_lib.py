@@ -126,6 +126,16 @@ _SIGS = {
                                           C.c_double, _ip, _dp]),
     "dwf_msbatch_refine_iterations": (C.c_int, [_vp]),
     "dwf_msbatch_destroy": (C.c_int, [_vp]),
+    "dwf_ptc_create": (C.c_int, [_vp, C.c_int, _ip, C.POINTER(_vp)]),
+    "dwf_ptc_destroy": (C.c_int, [_vp]),
+    "dwf_ptc_num_weights": (C.c_size_t, [_vp]),
+    "dwf_ptc_set_weights": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "dwf_ptc_get_weights": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "dwf_ptc_apply": (C.c_int, [_vp, _vp, _vp]),
+    "dwf_ptc_loss_grad": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]),
+    "dwf_solver_create_fcg_ptc": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.POINTER(_vp)]),
+    "dwf_solver_create_richardson_ptc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_int,
+                                                   C.c_double, C.POINTER(_vp)]),
     "dwf_force_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dwf_force_destroy": (C.c_int, [_vp]),
     "dwf_force_set_mode": (C.c_int, [_vp, C.c_char_p]),
