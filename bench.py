@@ -222,6 +222,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-ptc", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (the lattice "
@@ -369,6 +370,57 @@ def main():
     log(f"learned preconditioner: training {t_train:.3f} s (loss {lpc_loss:.4f}); "
         f"solve {t_lpc:.3f} s, outer={n_lpc}, inner={i_lpc}")
 
+    # learned parallel-transport network (4 layers, 8 channels, bfloat16 weights):
+    # trained on the GPU on an 8^3 x 16 lattice with the same action (2048
+    # (source, solution) pairs b = D^+D x), weights moved to the 16^3 x 32 network
+    ptc = None
+    if not a.no_ptc:
+        from dwf_amd.ptc import (ASolver_Richardson_PTC, ParallelTransportNet, operator_pairs,
+                                 train)
+        small = (8, 8, 8, 16)
+        sms = dwf_amd.Director_alt_Smear(small, RHO, NSMEAR)
+        fs = dwf_amd.AFopr_Domainwall_5din_eo(small, LS, MQ, M0, B, CC, (1, 1, 1, -1), "float")
+        fs.set_config_smeared(sms, lattice.random_gauge(small, EPS, 77))
+        fs.set_mode("DdagD")
+        ch = (1, 8, 8, 8, 1)
+        nets = ParallelTransportNet(fs, ch)
+        nets.set_weights(nets.identity_weights(scale=0.5, noise=1e-3, seed=1))
+        dwf_amd.synchronize()
+        t0 = time.perf_counter()
+        pairs = operator_pairs(fs, 2048, seed=5)
+        hist = train(nets, pairs, steps=500, batch=4, lr=2e-3, seed=3)
+        dwf_amd.synchronize()
+        t_ptc_train = time.perf_counter() - t0
+        del pairs
+        netf = ParallelTransportNet(f32, ch)
+        netf.set_weights(nets.get_weights())
+        # one application at full size, HIP events
+        w32, z32 = f32.new_field(), f32.new_field()
+        w32.convert_precision_from(b)
+        netf.apply(z32, w32)
+        ev.record(e0, st)
+        for _ in range(3):
+            netf.apply(z32, w32)
+        ev.record(e1, st)
+        ptc_apply_ms = ev.elapsed_ms(e0, e1) / 3
+        richp = ASolver_Richardson_PTC(f64, f32, netf, 100, TOL2, 100000, INNER_TOL2_FAST)
+        t_p, n_p, d_p, i_p = timed(richp)
+        log(f"learned PTC network: training {t_ptc_train:.1f} s (loss {hist[0]:.4f} -> "
+            f"{np.mean(hist[-20:]):.4f}); one application {ptc_apply_ms:.2f} ms; solve "
+            f"{t_p:.3f} s, outer={n_p}, inner={i_p}")
+        # the same Richardson / FCG with the identity network (= CG) for the iteration gain
+        netf.set_weights(nets.identity_weights(scale=0.5, noise=0.0))
+        t_p0, n_p0, d_p0, i_p0 = timed(ASolver_Richardson_PTC(f64, f32, netf, 100, TOL2, 100000,
+                                                               INNER_TOL2_FAST))
+        log(f"  same solver, identity network: {t_p0:.3f} s, outer={n_p0}, inner={i_p0}")
+        ptc = {"train_s": round(t_ptc_train, 2), "train_pairs": 2048, "train_steps": 500,
+               "train_lattice": list(small), "loss_first": round(hist[0], 5),
+               "loss_last20": round(float(np.mean(hist[-20:])), 5),
+               "apply_ms_full_size": round(ptc_apply_ms, 3),
+               "solve_s": round(t_p, 4), "outer_inner": [n_p, i_p],
+               "identity_network_solve_s": round(t_p0, 4), "identity_outer_inner": [n_p0, i_p0],
+               "inner_iteration_ratio": round(i_p / max(i_p0, 1), 3)}
+
     # multishift CG (the rational part): 16 shifts log-spaced on [1e-4, 10], FP64
     sigma = np.logspace(-4, 1, 16)
     xs = [f64.new_field() for _ in sigma]
@@ -435,6 +487,7 @@ def main():
                                       "ms_per_iteration": round(t_ms / max(n_ms, 1) * 1e3, 3),
                                       "max_rel_residual": d_ms},
         "multishift_16_shifts_4_rhs_batched": batch,
+        "learned_ptc_network": ptc,
         "operator_rel_error_vs_fp64": {k: float(f"{e:.3e}") for k, e in op_err.items()},
         # the hop the timed solve spends its time in: FP16 storage D_eo, algorithmic
         # bytes (16-bit spinors and links) per launch over its HIP-event time
