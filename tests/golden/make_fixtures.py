@@ -64,6 +64,33 @@ def main():
         assert text in row, (key, row)
         known[key] = {"value": float(row.split("=")[1]),
                       "source": "sample_1gpu/run.log:%d (%s)" % (line, row.strip())}
+    # the same input file on 32x16x16x24 (4 ranks, grid [1, 1, 2, 2]; the parallel
+    # generator's streams are per global (z, t), so one process draws the same numbers)
+    name = "submit_miyabi/submit_4node.sh.o1137366"
+    log = open(os.path.join(REF, name)).read().split("\n")
+    want = [
+        ("H_kin_init", 1515, "H_kin        ="),
+        ("H_Frational_init", 1526, "H_Frational  ="),
+        ("H_Fratio_Nf2_init", 1532, "H_Fratio     ="),
+        ("H_Fratio_Nf2prec_init", 1545, "H_Fratio     ="),
+        ("H_Grectangle_init", 1557, "H_Grectangle ="),
+        ("H_total_init", 1560, "H_total(init)  ="),
+        ("plaq_init", 1562, "plaq(init)     ="),
+        ("H_kin_trial", 1960, "H_kin        ="),
+        ("H_Frational_trial", 1971, "H_Frational  ="),
+        ("H_Fratio_Nf2_trial", 1978, "H_Fratio     ="),
+        ("H_Fratio_Nf2prec_trial", 1985, "H_Fratio     ="),
+        ("H_Grectangle_trial", 1990, "H_Grectangle ="),
+        ("H_total_trial", 1993, "H_total(trial) ="),
+        ("plaq_trial", 1995, "plaq(trial)    ="),
+        ("H_diff", 1997, "H(diff)       ="),
+        ("trajectory_seconds", 2003, "each trajectory: total"),
+    ]
+    for key, line, text in want:
+        row = log[line - 1]
+        assert text in row, (key, row)
+        v = row.split("total")[1].split("sec")[0] if key == "trajectory_seconds" else row.split("=")[1]
+        known[key + "_32x16x16x24"] = {"value": float(v), "source": "%s:%d (%s)" % (name, line, row.strip())}
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump(known, f, indent=1)
 

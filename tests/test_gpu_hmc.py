@@ -177,3 +177,37 @@ def test_config1_trajectory_8x8x8x8():
           f"plaq {out['plaq_init']:.10f} -> {out['plaq_trial']:.10f}", flush=True)
     assert np.isfinite(out["H_diff"])
     assert abs(out["H_diff"]) < 1e-4 * abs(out["H_total_init"])
+
+
+def test_reference_trajectory_32x16x16x24():
+    """The same input file on 32x16x16x24, run by the reference on 4 ranks
+    (submit_4node.sh.o1137366:1515-1997; the parallel generator's streams are per
+    global (z, t), so one GPU draws the same numbers): the Langevin energies to
+    1e-10 relative, the trial energies to 1e-11 relative (about 1e-4 absolute on
+    terms of 1.4e7, the precision the stopping rule 1e-28 leaves in |phi^+ M^-1 phi|)
+    and H(diff) = 0.05636821 to 1e-6 absolute."""
+    import time
+    dims = (32, 16, 16, 24)
+    sfx = "_32x16x16x24"
+    hmcobj, rng = hmc.benchmark_nf2p1(dims, RATIONAL)
+    rng.reset(200000)
+    U0 = lattice.tile_4x4x4x8(np.load(os.path.join(HERE, "golden", "conf_04040408.npy")), dims)
+    U = dev(U0, dims)
+    print("case: reference 32x16x16x24 trajectory (HMC_DWF_Nf2p1_eo.yaml)", flush=True)
+    t0 = time.perf_counter()
+    out = hmcobj.update(U)
+    dt = time.perf_counter() - t0
+    print(f"  {dt:.2f} s (reference, 4 ranks: {KNOWN['trajectory_seconds' + sfx]['value']} s)"
+          f"\n  init {out['init']}\n  trial {out['trial']}\n  H_diff {out['H_diff']:.10f}",
+          flush=True)
+    names = [("H_kin", "H_kin"), ("H_0_Frational", "H_Frational"),
+             ("H_1_Fratio_Nf2", "H_Fratio_Nf2"), ("H_2_Fratio_Nf2prec", "H_Fratio_Nf2prec"),
+             ("H_3_gauge", "H_Grectangle")]
+    for mine, ref in names:
+        want = KNOWN[ref + "_init" + sfx]["value"]
+        assert abs(out["init"][mine] - want) / want < 1e-10, (mine, out["init"][mine], want)
+        want = KNOWN[ref + "_trial" + sfx]["value"]
+        assert abs(out["trial"][mine] - want) / want < 1e-11, (mine, out["trial"][mine], want)
+    assert abs(out["plaq_init"] - KNOWN["plaq_init" + sfx]["value"]) < 1e-12
+    assert abs(out["plaq_trial"] - KNOWN["plaq_trial" + sfx]["value"]) < 1e-11
+    assert abs(out["H_diff"] - KNOWN["H_diff" + sfx]["value"]) < 1e-6

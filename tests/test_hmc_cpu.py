@@ -44,6 +44,24 @@ def test_rng_momenta_and_noise_match_reference():
         assert abs(h - KNOWN[key]["value"]) / KNOWN[key]["value"] < 1e-12, (key, h)
 
 
+def test_rng_matches_reference_on_32x16x16x24():
+    """The same input on 32x16x16x24 ran on 4 ranks (grid [1, 1, 2, 2]): the
+    parallel generator's streams belong to global (z, t) slices, so this single
+    process must draw the numbers of that run (submit_4node.sh.o1137366:1515-1545)."""
+    dims = (32, 16, 16, 24)
+    r = RandomNumbers_parallel_Mseries(dims, 1234567)
+    r.reset(200000)
+    P, H_kin = set_iP(r.gauss_lex_global(8, 4))
+    del P
+    assert abs(H_kin - KNOWN["H_kin_init_32x16x16x24"]["value"]) < 1e-6
+    for key in ("H_Frational_init", "H_Fratio_Nf2_init", "H_Fratio_Nf2prec_init"):
+        xi = even_part(r.gauss_lex_global(24, 12), dims)
+        h = float(np.sum(xi * xi))
+        want = KNOWN[key + "_32x16x16x24"]["value"]
+        # H_F = |xi|^2 to the solver's precision (the reference prints |phi^+ M^-1 phi|)
+        assert abs(h - want) / want < 1e-9, (key, h, want)
+
+
 def test_rng_reset_restarts_the_streams():
     r = RandomNumbers_parallel_Mseries((4, 4, 4, 4), 1)
     r.reset(7)
