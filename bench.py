@@ -223,6 +223,7 @@ def main():
     ap.add_argument("--no-fp64-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-ptc", action="store_true")
+    ap.add_argument("--no-hmc", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (the lattice "
@@ -438,6 +439,22 @@ def main():
     if not a.no_batch:
         batch = bench_batch(dwf_amd, lattice, np, f64, f32, sm, log)
 
+    # the reference's 2+1-flavour trajectory on 32x16x16x24 (HMC_DWF_Nf2p1_eo.yaml),
+    # whose energies and time the reference printed for 4 GPUs
+    traj = None
+    if not a.no_hmc:
+        from dwf_amd.benchmark_hmc_dwf import run_trajectory
+        known = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
+        ref_s = known["trajectory_seconds_32x16x16x24"]["value"]
+        ref_dh = known["H_diff_32x16x16x24"]["value"]
+        o = run_trajectory((32, 16, 16, 24))
+        traj = {"seconds": round(o["seconds"], 3), "H_diff": o["H_diff"],
+                "reference_H_diff": ref_dh, "abs_H_diff_error": abs(o["H_diff"] - ref_dh),
+                "reference_seconds_4_gpus": ref_s,
+                "speedup_vs_reference_4_gpus": round(ref_s / o["seconds"], 3)}
+        log(f"HMC trajectory 32x16x16x24 Ls=12: {o['seconds']:.2f} s, H(diff) = "
+            f"{o['H_diff']:.8f} (reference {ref_dh}, {ref_s} s on 4 GPUs)")
+
     cpu = None
     if not a.no_cpu_baseline:
         Us = sm.get_config(NSMEAR)
@@ -488,6 +505,7 @@ def main():
                                       "max_rel_residual": d_ms},
         "multishift_16_shifts_4_rhs_batched": batch,
         "learned_ptc_network": ptc,
+        "hmc_trajectory_32x16x16x24_Ls12": traj,
         "operator_rel_error_vs_fp64": {k: float(f"{e:.3e}") for k, e in op_err.items()},
         # the hop the timed solve spends its time in: FP16 storage D_eo, algorithmic
         # bytes (16-bit spinors and links) per launch over its HIP-event time

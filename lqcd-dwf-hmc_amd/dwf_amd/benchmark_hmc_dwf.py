@@ -1,0 +1,70 @@
+"""The 2+1-flavour domain-wall HMC benchmark driver on one GPU.
+
+The counterpart of benchmark_hmc_dwf/hmc_dwf_eo.cpp:180-360 run with
+HMC_DWF_Nf2p1_eo.yaml: the gauge field conf_04040408.txt (4^3 x 8) tiled
+periodically onto the lattice, RandomNumbers_parallel<Mseries> seeded with
+1234567 and reset to 200000 before the trajectory, one trajectory of length
+0.1 with Omelyan steps [2, 1, 3], no Metropolis test.  Prints the energies the
+reference prints (H_kin, H_F per action, H_Grectangle, H_total, plaq, H(diff))
+and the trajectory's wall time.
+
+    python lqcd-dwf-hmc_amd/dwf_amd/benchmark_hmc_dwf.py --lattice 32 16 16 24
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(os.path.dirname(PKG), "tests", "golden")
+
+
+def run_trajectory(lattice_size, conf_npy=None, rational_dir=None, i_conf=200000,
+                   log=None):
+    """One trajectory of the benchmark; returns the HMC_General.update result
+    with "seconds" (wall time of the trajectory, links already on the GPU)."""
+    import numpy as np
+    import dwf_amd
+    from dwf_amd import hmc, lattice
+    conf_npy = conf_npy or os.path.join(GOLDEN, "conf_04040408.npy")
+    rational_dir = rational_dir or os.path.join(GOLDEN, "rational")
+    dims = tuple(int(v) for v in lattice_size)
+    hmcobj, rng = hmc.benchmark_nf2p1(dims, rational_dir)
+    U = hmc.GaugeField(dims)
+    U.from_host(lattice.tile_4x4x4x8(np.load(conf_npy), dims))
+    rng.reset(i_conf)
+    dwf_amd.synchronize()
+    t0 = time.perf_counter()
+    out = hmcobj.update(U)
+    dwf_amd.synchronize()
+    out["seconds"] = time.perf_counter() - t0
+    out.pop("U_trial", None)
+    if log:
+        for k, v in out["init"].items():
+            log(f"  {k:22s} = {v:20.8f}")
+        log(f"  H_total(init)          = {out['H_total_init']:.10f}")
+        log(f"  plaq(init)             = {out['plaq_init']:.12f}")
+        for k, v in out["trial"].items():
+            log(f"  {k:22s} = {v:20.8f}")
+        log(f"  H_total(trial)         = {out['H_total_trial']:.10f}")
+        log(f"  plaq(trial)            = {out['plaq_trial']:.12f}")
+        log(f"  H(diff)                = {out['H_diff']:.8f}")
+        log(f"  trajectory: {out['seconds']:.2f} s")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lattice", type=int, nargs=4, default=[8, 8, 8, 8])
+    ap.add_argument("--conf", default=None, help="site-major (V, 4, 18) .npy of the 4^3x8 field")
+    ap.add_argument("--rational", default=None, help="folder of parameter_rational.* files")
+    a = ap.parse_args()
+    sys.path.insert(0, PKG)
+    run_trajectory(a.lattice, a.conf, a.rational,
+                   log=lambda s: print(s, flush=True))
+
+
+if __name__ == "__main__":
+    main()
