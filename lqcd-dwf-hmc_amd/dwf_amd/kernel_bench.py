@@ -3,11 +3,14 @@ of known size, the calibration for hardware byte counters).
 
     python3 lqcd-dwf-hmc_amd/dwf_amd/kernel_bench.py [--mode Deo] [--prec double]
                                                      [--reps 50] [--dims 16 16 16 32] [--Ls 16]
+                                                     [--lpc-train DEGREE]
 
 Runs <reps> launches of the operator mode (and <reps> field copies) on the
 library stream, timed with HIP events, and prints one line per measurement.
 The workload is the benchmark lattice: links exp(0.3 i H), Shamir b = 1, c = 0,
-mq = 0.01, M0 = 1.6, antiperiodic in t.
+mq = 0.01, M0 = 1.6, antiperiodic in t.  --lpc-train also trains the learned
+polynomial preconditioner of that degree once on the operator (its Gram
+matrices run on the matrix cores for an FP32 operator at Ls <= 16).
 """
 from __future__ import annotations
 
@@ -57,6 +60,7 @@ def main():
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--dims", type=int, nargs=4, default=[16, 16, 16, 32])
     ap.add_argument("--Ls", type=int, default=16)
+    ap.add_argument("--lpc-train", type=int, default=0)
     a = ap.parse_args()
     dims = tuple(a.dims)
     op = dwf_amd.AFopr_Domainwall_5din_eo(dims, a.Ls, 0.01, 1.6, 1.0, 0.0, (1, 1, 1, -1), a.prec)
@@ -67,10 +71,19 @@ def main():
     st = dwf_amd.stream()
     elem = {"double": 8, "float": 4}.get(a.prec, 2)
     field_bytes = op.Vh * a.Ls * 24 * elem
-    us = ev.time(lambda: op.mult(v, w, a.mode) if not a.dag else op.mult_dag(v, w, a.mode),
-                 a.reps, st)
+    if a.mode in ("Deo", "Doe", "Dee", "Doo", "Dee_inv", "Doo_inv"):
+        run = (lambda: op.mult_dag(v, w, a.mode)) if a.dag else (lambda: op.mult(v, w, a.mode))
+    else:   # D, Ddag, DdagD, H, ...: the operator's current mode
+        op.set_mode(a.mode)
+        run = (lambda: op.mult_dag(v, w)) if a.dag else (lambda: op.mult(v, w))
+    us = ev.time(run, a.reps, st)
     print(f"mode {a.mode}{'^dag' if a.dag else ''} prec {a.prec} dims {dims} Ls {a.Ls}: "
           f"{us:.2f} us/launch", flush=True)
+    if a.lpc_train:
+        lpc = dwf_amd.APrecond_LearnedPolynomial(op, a.lpc_train)
+        loss = lpc.train(op, nvec=2)
+        dwf_amd.synchronize()
+        print(f"learned polynomial degree {a.lpc_train} trained: loss {loss:.4f}", flush=True)
     us = ev.time(lambda: v.copy_from(w), a.reps, st)
     print(f"copy of one field ({field_bytes} bytes read, {field_bytes} written): "
           f"{us:.2f} us/launch, {2 * field_bytes / us / 1e3:.0f} GB/s", flush=True)
