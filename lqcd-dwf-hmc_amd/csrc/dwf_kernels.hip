@@ -1297,10 +1297,9 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
     const size_t epi = mat ? (size_t)sp * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
-  // short Ls: fewer sites per block so that the compiled-Ls kernel's link
-  // staging fits (FP32 at Ls = 4 would stage 128 sites), whole waves kept
-  if (lsc && Ls != 16)
-    while (lds_for(spb) > lds_cap && spb % 2 == 0 && (spb / 2) * (Ls / vec) % 64 == 0) spb /= 2;
+  // The compiled-Ls kernels fix their block at THREADS / (Ls / VEC) sites; when
+  // that block's link staging exceeds the LDS cap (FP64 and FP32 at Ls = 4) the
+  // runtime-Ls kernel below runs instead, with fewer sites per block.
   // the fused CG updates, and an output that is not stored, exist in the
   // matrix-core epilogue only
   const bool mfma_path = mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap;
