@@ -1876,6 +1876,18 @@ int dwf_field_fill_uniform(dwf_fopr op, void* x, unsigned long long seed) {
 }
 
 // AShiftsolver_CG::solve (ashiftsolver_CG-tmpl.h:89-329), FP64
+// The iteration of AShiftsolver_CG (ashiftsolver_CG-tmpl.h:89-286): the
+// unshifted CG (shift 0) runs as in the reference; the other shifts' vectors
+// are not touched every iteration.  With the residuals r_1 .. r_m of a block of
+// m <= kFoldMax iterations kept in a ring, x_j and p_j after the block are
+//   p_j = a p_j' + sum_l c_l r_l,   x_j = x_j' + b p_j' + sum_l d_l r_l
+// (primes: at the block start), where a, b, c, d follow from the reference's
+// scalar recurrences (x_j -= beta_s p_j; p_j = alpha_s p_j + zeta r); one fold
+// pass per block applies them (k_ms_fold), reading each r_l once for all
+// shifts: 1 + 4 n_shift / m field passes per iteration instead of 4 n_shift + 1.
+// The truncation rule needs |p_j|^2 every iteration: inside a block it follows
+// |p_j|^2 <- alpha_s^2 |p_j|^2 + zeta^2 |r|^2 (r is orthogonal to the previous
+// p_j in CG), and every fold replaces it by the computed norm.
 int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int nshift,
                              const double* sigma, void** xq, const void* b, int* nconv_out,
                              double* diff_out) {
@@ -1886,22 +1898,26 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
   RC(ensure_shift_scratch(nshift));
   const size_t n = op->nelem;
   const size_t bytes = n * sizeof(double);
+  const int K = kFoldMax;
   hipStream_t st = g_ctx.stream;
-  std::vector<void*> p(nshift, nullptr);
-  void *s = nullptr, *r = nullptr;
+  std::vector<void*> p(nshift, nullptr), R(K + 1, nullptr);
+  void *s = nullptr, *coef = nullptr, *rptr = nullptr;
   auto release = [&]() {
     (void)hipStreamSynchronize(st);
     for (auto& q : p)
       if (q) (void)hipFree(q);
+    for (auto& q : R)
+      if (q) (void)hipFree(q);
     if (s) (void)hipFree(s);
-    if (r) (void)hipFree(r);
+    if (coef) (void)hipFree(coef);
+    if (rptr) (void)hipFree(rptr);
   };
-  for (int i = 0; i < nshift; ++i)
-    if (hipMalloc(&p[i], bytes) != hipSuccess) {
-      release();
-      return fail(DWF_ERR_HIP, "device allocation failed");
-    }
-  if (hipMalloc(&s, bytes) != hipSuccess || hipMalloc(&r, bytes) != hipSuccess) {
+  bool ok = hipMalloc(&s, bytes) == hipSuccess &&
+            hipMalloc(&coef, sizeof(double) * kFoldCoef * nshift) == hipSuccess &&
+            hipMalloc(&rptr, sizeof(void*) * K) == hipSuccess;
+  for (int i = 0; ok && i < nshift; ++i) ok = hipMalloc(&p[i], bytes) == hipSuccess;
+  for (int i = 0; ok && i <= K; ++i) ok = hipMalloc(&R[i], bytes) == hipSuccess;
+  if (!ok) {
     release();
     return fail(DWF_ERR_HIP, "device allocation failed");
   }
@@ -1916,13 +1932,11 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
     for (int i = 0; i < nshift; ++i) csh2[i] = sigma[i] - sigma[0];
     const double sigma0 = sigma[0];
     int Nshift2 = nshift;
-    HIPCHK(launch_copy(P64, s, b, n, st));
-    HIPCHK(launch_copy(P64, r, b, n, st));
+    HIPCHK(launch_copy(P64, R[0], b, n, st));
     for (int i = 0; i < nshift; ++i) {
-      HIPCHK(launch_copy(P64, p[i], s, n, st));
+      HIPCHK(launch_copy(P64, p[i], b, n, st));
       HIPCHK(launch_set_zero(P64, xq[i], n, st));
     }
-    // device pointer arrays for the batched shift update
     void** hp = (void**)(g_ctx.host_coef + 3 * nshift);
     for (int i = 0; i < nshift; ++i) {
       hp[i] = xq[i];
@@ -1930,9 +1944,39 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
     }
     HIPCHK(hipMemcpyAsync(g_ctx.sh_ptrs, hp, sizeof(void*) * 2 * nshift, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
-    double rr;
-    HIPCHK(launch_norm2(P64, r, n, g_ctx.red, 0, st));
-    RC(fetch(1, &rr));
+    double rr = bn;
+    // block coefficients per shift: [a, b, c_1..c_K, d_1..d_K]; |p_j|^2 estimates
+    std::vector<double> cf((size_t)kFoldCoef * nshift, 0.0), ppe(nshift, bn);
+    auto reset_coef = [&]() {
+      std::fill(cf.begin(), cf.end(), 0.0);
+      for (int j = 0; j < nshift; ++j) cf[(size_t)j * kFoldCoef] = 1.0;
+    };
+    reset_coef();
+    int m = 0;            // iterations in the current block (R[1..m] hold their residuals)
+    int nfold = nshift;   // shifts 1..nfold-1 were active at the block start
+    auto fold = [&]() -> int {
+      if (nfold > 1 && m > 0) {
+        HIPCHK(hipMemcpyAsync(coef, cf.data(), sizeof(double) * kFoldCoef * nfold,
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(rptr, R.data() + 1, sizeof(void*) * m, hipMemcpyHostToDevice, st));
+        // the fold kernel takes up to 31 shifts (local 1..31) per pass
+        for (int j0 = 0; j0 + 1 < nfold; j0 += 31) {
+          const int nloc = std::min(32, nfold - j0);
+          HIPCHK(launch_ms_fold(g_ctx.sh_ptrs + j0, g_ctx.sh_ptrs + nshift + j0,
+                                (const void* const*)rptr, (const double*)coef + (size_t)j0 * kFoldCoef,
+                                nloc, m, n, g_ctx.sh_partial, g_ctx.sh_pp + j0, st));
+        }
+        double* hpp = g_ctx.host_coef + 5 * nshift;
+        HIPCHK(hipMemcpyAsync(hpp, g_ctx.sh_pp, sizeof(double) * nfold, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int j = 1; j < nfold; ++j) ppe[j] = hpp[j];
+      }
+      std::swap(R[0], R[m]);   // the block's last residual starts the next block
+      reset_coef();
+      m = 0;
+      nfold = Nshift2;
+      return DWF_OK;
+    };
     double alpha_p = 0.0, beta_p = 1.0;
     int Nconv2 = -1;
     bool converged = false;
@@ -1943,40 +1987,30 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
       RC(fetch(1, &pa_p));
       const double rr_p = rr;
       const double beta = -rr_p / pa_p;
-      HIPCHK(launch_cg_xr(P64, xq[0], r, p[0], s, -beta, n, g_ctx.red, 0, st));
+      ++m;
+      HIPCHK(launch_cg_xr_to(P64, xq[0], R[m], R[m - 1], p[0], s, -beta, n, g_ctx.red, 0, st));
       RC(fetch(1, &rr));
       const double alpha = rr / rr_p;
-      HIPCHK(launch_aypx(P64, alpha, p[0], r, n, st));
+      HIPCHK(launch_aypx(P64, alpha, p[0], R[m], n, st));
       pp[0] = rr;
       const double alpha_h = 1.0 + alpha_p * beta / beta_p;
-      std::vector<double> znew(nshift, 0.0);
-      double* hc = g_ctx.host_coef;
       for (int ish = 1; ish < Nshift2; ++ish) {
         double zeta = (alpha_h - csh2[ish] * beta) / zeta1[ish] + (1.0 - alpha_h) / zeta2[ish];
         zeta = 1.0 / zeta;
         const double zr = zeta / zeta1[ish];
         const double beta_s = beta * zr;
         const double alpha_s = alpha * zr * zr;
-        hc[3 * ish] = -beta_s;
-        hc[3 * ish + 1] = alpha_s;
-        hc[3 * ish + 2] = zeta;
-        znew[ish] = zeta;
-      }
-      if (Nshift2 > 1) {
-        HIPCHK(hipMemcpyAsync(g_ctx.sh_coef, hc, sizeof(double) * 3 * Nshift2,
-                              hipMemcpyHostToDevice, st));
-        HIPCHK(launch_shift_update(g_ctx.sh_ptrs, g_ctx.sh_ptrs + nshift, r, g_ctx.sh_coef,
-                                   Nshift2, n, g_ctx.sh_partial, g_ctx.sh_counters, g_ctx.sh_pp,
-                                   st));
-        double* hpp = g_ctx.host_coef + 5 * nshift;
-        HIPCHK(hipMemcpyAsync(hpp, g_ctx.sh_pp, sizeof(double) * Nshift2,
-                              hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int ish = 1; ish < Nshift2; ++ish) {
-          pp[ish] = hpp[ish] * snorm;
-          zeta2[ish] = zeta1[ish];
-          zeta1[ish] = znew[ish];
-        }
+        double* c = cf.data() + (size_t)ish * kFoldCoef;
+        // x_j -= beta_s p_j (p_j before its update), then p_j = alpha_s p_j + zeta r_m
+        c[1] -= beta_s * c[0];
+        for (int l = 0; l < m - 1; ++l) c[2 + K + l] -= beta_s * c[2 + l];
+        c[0] *= alpha_s;
+        for (int l = 0; l < m - 1; ++l) c[2 + l] *= alpha_s;
+        c[2 + m - 1] = zeta;
+        ppe[ish] = alpha_s * alpha_s * ppe[ish] + zeta * zeta * rr;
+        pp[ish] = ppe[ish] * snorm;
+        zeta2[ish] = zeta1[ish];
+        zeta1[ish] = zeta;
       }
       int ish1 = Nshift2;
       for (int ish = Nshift2 - 1; ish >= 0; --ish)
@@ -1988,11 +2022,11 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
       alpha_p = alpha;
       beta_p = beta;
       Nconv2 += 1;
-      if (rr * snorm < stop_cond) {
-        converged = true;
-        break;
-      }
+      converged = rr * snorm < stop_cond;
+      if (m == K || converged) RC(fold());
+      if (converged) break;
     }
+    if (!converged && m > 0) RC(fold());
     double dmax = -1.0;
     for (int i = 0; i < nshift; ++i) {
       RC(mult_impl(op, s, xq[i], op->mode, false));

@@ -202,6 +202,10 @@ hipError_t launch_dot(int prec, const void* x, const void* y, size_t n, const Re
 hipError_t launch_dotc_ls(int prec, const void* x, const void* y, size_t n, int Ls,
                           const RedScratch& r, int slot, hipStream_t st);
 // CG step:  x += a p;  r -= a s;  result[slot] = |r|^2
+// x += a p, r_out = r_in - a s, |r_out|^2 -> result[slot] (r_out != r_in)
+hipError_t launch_cg_xr_to(int prec, void* x, void* rout, const void* rin, const void* p,
+                           const void* s, double a, size_t n, const RedScratch& red, int slot,
+                           hipStream_t st);
 hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s, double a,
                         size_t n, const RedScratch& red, int slot, hipStream_t st);
 // y += a x; result[slot] = Re<y, x>   (used as s += sigma0 p; pap = <s, p>)
@@ -228,6 +232,16 @@ hipError_t launch_cg_px_dev(int prec, void* x, void* p, const void* r, const dou
 hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
                                const double* coef, int m, size_t n, double* partial,
                                unsigned* counters, double* pp, hipStream_t st);
+
+// Krylov-block fold of the multishift CG for shifts j = 1..nsh-1, with the
+// block's residuals R[0..m) (m <= kFoldMax) and per-shift coefficients
+// [a, b, c_1..c_K, d_1..d_K] (kFoldCoef doubles, K = kFoldMax) on the device:
+//   x_j += b p_j + sum_l d_l R_l;  p_j = a p_j + sum_l c_l R_l;  pp[j] = |p_j|^2
+constexpr int kFoldMax = 16;
+constexpr int kFoldCoef = 2 + 2 * kFoldMax;
+hipError_t launch_ms_fold(void* const* x, void* const* p, const void* const* R,
+                          const double* coef, int nsh, int m, size_t n, double* partial,
+                          double* pp, hipStream_t st);
 
 // v = a0 b + sum_{j<m} c[j] x[j] (double; x and c device arrays)
 hipError_t launch_lincomb(void* v, const void* b, double a0, void* const* x, const double* c,

@@ -2075,6 +2075,37 @@ k_cg_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* 
   }
   block_finish<1>(v, partial, counter, result);
 }
+// x += a p;  r_out = r_in - a s (r_out another buffer);  |r_out|^2
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_cg_xr_to(T* __restrict__ x, T* __restrict__ rout, const T* __restrict__ rin,
+           const T* __restrict__ p, const T* __restrict__ s, typename St<T>::C a, size_t n,
+           double* partial, unsigned* counter, double* result) {
+  using C = typename St<T>::C;
+  double v[1] = {0};
+  PACK_LOOP(T, n) {
+    C xv[NV], rv[NV], pv[NV], sv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(p + ip * NV, pv);
+    Vec<T>::ld(rin + ip * NV, rv);
+    Vec<T>::ld(s + ip * NV, sv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      xv[j] += a * pv[j];
+      rv[j] = rounded<T>(rv[j] - a * sv[j]);
+      v[0] += (double)rv[j] * (double)rv[j];
+    }
+    Vec<T>::st(x + ip * NV, xv);
+    Vec<T>::st(rout + ip * NV, rv);
+  }
+  TAIL_LOOP(n) {
+    St<T>::st(x + i, St<T>::ld(x + i) + a * St<T>::ld(p + i));
+    C rn = rounded<T>(St<T>::ld(rin + i) - a * St<T>::ld(s + i));
+    St<T>::st(rout + i, rn);
+    v[0] += (double)rn * (double)rn;
+  }
+  block_finish<1>(v, partial, counter, result);
+}
 // y += a x;  Re<y, x> of the stored y
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -2148,6 +2179,14 @@ static void cg_xr_t(void* x, void* rr, const void* p, const void* s, double a, s
   red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 template <typename T>
+static void cg_xr_to_t(void* x, void* rout, const void* rin, const void* p, const void* s,
+                       double a, size_t n, const RedScratch& r, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_xr_to<T>, red_grid(n), dim3(256), 0, st, (T*)x, (T*)rout,
+                     (const T*)rin, (const T*)p, (const T*)s, (typename St<T>::C)a, n, r.partial,
+                     r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
+}
+template <typename T>
 static void axpy_dot_t(void* y, double a, const void* x, size_t n, const RedScratch& r, int slot,
                        hipStream_t st) {
   hipLaunchKernelGGL(k_axpy_dot<T>, red_grid(n), dim3(256), 0, st, (T*)y, (typename St<T>::C)a,
@@ -2188,6 +2227,13 @@ hipError_t launch_dotc_ls(int prec, const void* x, const void* y, size_t n, int 
 hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s, double a,
                         size_t n, const RedScratch& red, int slot, hipStream_t st) {
   DWF_DISPATCH(prec, cg_xr_t, (x, r, p, s, a, n, red, slot, st));
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_xr_to(int prec, void* x, void* rout, const void* rin, const void* p,
+                           const void* s, double a, size_t n, const RedScratch& red, int slot,
+                           hipStream_t st) {
+  DWF_DISPATCH(prec, cg_xr_to_t, (x, rout, rin, p, s, a, n, red, slot, st));
   return hipGetLastError();
 }
 
@@ -2422,6 +2468,72 @@ hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
   hipLaunchKernelGGL(k_shift_update, g, dim3(256), 0, st, (double* const*)x, (double* const*)p,
                      (const double*)r, coef, n, partial, counters, pp);
   red_finish<1>(partial, g.x, pp + 1, st, g.y, kRedBlocks, 1);
+  return hipGetLastError();
+}
+
+// Krylov-block fold of the multishift CG.  After m <= kFoldMax iterations
+// whose unshifted residuals are R_1 .. R_m, shift j's vectors are linear
+// combinations of their values at the start of the block and the R_l:
+//   x_j <- x_j + b_j p_j + sum_l d_jl R_l,   p_j <- a_j p_j + sum_l c_jl R_l,
+// coef per shift j (1 <= j < nsh): [a, b, c_1 .. c_K, d_1 .. d_K], K = kFoldMax,
+// zero beyond m.  One pass reads each R_l once for all shifts (instead of r
+// and x_j, p_j of every shift every iteration); |p_j|^2 per shift as block
+// partials (region j - 1 of kRedBlocks), as k_shift_update_all.
+__global__ void __launch_bounds__(256)
+k_ms_fold(double* const* __restrict__ x, double* const* __restrict__ p,
+          const double* const* __restrict__ R, const double* __restrict__ coef, int nsh, int m,
+          size_t n, double* partial) {
+  double v[kShiftMax];
+#pragma unroll
+  for (int j = 0; j < kShiftMax; ++j) v[j] = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double rv[kFoldMax];
+#pragma unroll
+    for (int l = 0; l < kFoldMax; ++l) rv[l] = l < m ? R[l][i] : 0.0;
+#pragma unroll
+    for (int j = 1; j < kShiftMax; ++j) {
+      if (j < nsh) {
+        const double* c = coef + j * kFoldCoef;
+        double* pj = p[j];
+        double* xj = x[j];
+        const double pv = pj[i];
+        double pn = c[0] * pv, xn = xj[i] + c[1] * pv;
+#pragma unroll
+        for (int l = 0; l < kFoldMax; ++l) {
+          pn += c[2 + l] * rv[l];
+          xn += c[2 + kFoldMax + l] * rv[l];
+        }
+        xj[i] = xn;
+        pj[i] = pn;
+        v[j] += pn * pn;
+      }
+    }
+  }
+  __shared__ double sh[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 1; j < kShiftMax; ++j) {
+    if (j < nsh) {
+      const double t = wave_sum(v[j]);
+      if (lane == 0) sh[wid] = t;
+      __syncthreads();
+      if (threadIdx.x == 0)
+        partial[(size_t)(j - 1) * kRedBlocks + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+      __syncthreads();
+    }
+  }
+}
+
+hipError_t launch_ms_fold(void* const* x, void* const* p, const void* const* R,
+                          const double* coef, int nsh, int m, size_t n, double* partial,
+                          double* pp, hipStream_t st) {
+  if (nsh <= 1) return hipSuccess;
+  if (nsh > kShiftMax || m < 1 || m > kFoldMax) return hipErrorInvalidValue;
+  const dim3 g = red_grid(n);
+  hipLaunchKernelGGL(k_ms_fold, g, dim3(256), 0, st, (double* const*)x, (double* const*)p,
+                     (const double* const*)R, coef, nsh, m, n, partial);
+  red_finish<1>(partial, g.x, pp + 1, st, nsh - 1, kRedBlocks, 1);
   return hipGetLastError();
 }
 
