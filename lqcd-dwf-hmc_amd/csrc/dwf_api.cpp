@@ -2061,7 +2061,7 @@ struct MsMember {
   dwf_fopr_s* op = nullptr;
   int nrs = 0, npad = 1;          // 2^nrs fields per buffer, of which the first nr are systems
   std::vector<int> sys;           // system index of each used slot
-  void* buf[7] = {};              // P0 S R T0 T1 U U2, each npad fields
+  void* buf[7] = {};              // P0 S (unused) T0 T1 U U2, each npad fields
 };
 enum { MB_P0, MB_S, MB_R, MB_T0, MB_T1, MB_U, MB_U2 };
 
@@ -2072,6 +2072,7 @@ struct dwf_msbatch_s {
   std::vector<MsMember> mem;
   std::vector<std::pair<int, int>> slot;   // system -> (member, slot)
   std::vector<std::vector<void*>> p;       // shift search directions p_kj, j >= 1
+  std::vector<std::vector<void*>> ring;    // residual ring per system (kFoldMax + 1 fields)
   // operator stages: per stage and member the launch, or one grouped launch
   std::vector<std::vector<GroupMember>> stages;
   std::vector<int> stage_op;
@@ -2179,6 +2180,9 @@ static void ms_release(dwf_msbatch_s* m) {
   for (auto& v : m->p)
     for (auto& q : v)
       if (q) (void)hipFree(q);
+  for (auto& v : m->ring)
+    for (auto& q : v)
+      if (q) (void)hipFree(q);
   if (m->dtab) (void)hipFree(m->dtab);
   if (m->dcounter) (void)hipFree(m->dcounter);
   if (m->L.state) (void)hipFree(m->L.state);
@@ -2246,7 +2250,7 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
     M.npad = 1 << M.nrs;
     const size_t fb = M.op->nelem * prec_bytes(M.op->prec) * M.npad;
     for (int b = 0; b < 7; ++b) {
-      if (b == MB_T0 && m->mp_scalar) continue;
+      if ((b == MB_T0 && m->mp_scalar) || b == MB_R) continue;
       if (hipMalloc(&M.buf[b], fb) != hipSuccess) return bad(DWF_ERR_HIP, "device allocation failed");
       if (hipMemset(M.buf[b], 0, fb) != hipSuccess) return bad(DWF_ERR_HIP, "hipMemset failed");
     }
@@ -2257,6 +2261,11 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
     for (int j = 1; j < nshift; ++j)
       if (hipMalloc(&m->p[k][j], m->ops[k]->nelem * prec_bytes(m->prec)) != hipSuccess)
         return bad(DWF_ERR_HIP, "device allocation failed");
+  m->ring.assign(nsys, std::vector<void*>(kFoldMax + 1, nullptr));
+  for (int k = 0; k < nsys; ++k)
+    for (int q = 0; q <= kFoldMax; ++q)
+      if (hipMalloc(&m->ring[k][q], m->ops[k]->nelem * prec_bytes(m->prec)) != hipSuccess)
+        return bad(DWF_ERR_HIP, "device allocation failed");
   ms_build_stages(m);
   if (m->mem.size() > 1) {
     m->glaunch.resize(m->stages.size());
@@ -2265,8 +2274,10 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
                             m->stages[i].data(), &m->glaunch[i]) != hipSuccess)
         return bad(DWF_ERR_UNSUPPORTED, "grouped launch could not be prepared");
   }
-  // device tables: s p0 x0 r [nsys], x p [nsys][32], n [nsys]
-  const size_t ptab = sizeof(void*) * (4 * nsys + 2 * nsys * kMsShifts);
+  // device tables: s p0 x0 [nsys], ring [nsys][kFoldMax + 1], x p [nsys][32], n [nsys]
+  constexpr int NR = kFoldMax + 1;
+  const size_t ntab = 3 * nsys + NR * nsys + 2 * nsys * kMsShifts;
+  const size_t ptab = sizeof(void*) * ntab;
   if (hipMalloc(&m->dtab, ptab + sizeof(size_t) * nsys) != hipSuccess)
     return bad(DWF_ERR_HIP, "device allocation failed");
   size_t maxn = 0;
@@ -2278,9 +2289,9 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
   m->L.s = t;
   m->L.p0 = t + nsys;
   m->L.x0 = t + 2 * nsys;
-  m->L.r = t + 3 * nsys;
-  m->L.x = t + 4 * nsys;
-  m->L.p = t + 4 * nsys + nsys * kMsShifts;
+  m->L.ring = t + 3 * nsys;
+  m->L.x = t + 3 * nsys + NR * nsys;
+  m->L.p = t + 3 * nsys + NR * nsys + nsys * kMsShifts;
   m->L.n = (const size_t*)((char*)m->dtab + ptab);
   if (hipMalloc(&m->L.state, sizeof(MsState) * nsys) != hipSuccess ||
       hipMalloc(&m->L.partial, sizeof(double) * nsys * kMsShifts * m->L.nblk) != hipSuccess ||
@@ -2296,15 +2307,16 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
   m->L.status = m->hstatus;
   m->L.counter = m->dcounter;
   // fixed part of the pointer tables (the member buffers)
-  std::vector<void*> h(4 * nsys + 2 * nsys * kMsShifts, nullptr);
+  std::vector<void*> h(ntab, nullptr);
   std::vector<size_t> hn(nsys);
   for (int k = 0; k < nsys; ++k) {
     const MsMember& M = m->mem[m->slot[k].first];
     const size_t off = (size_t)m->slot[k].second * M.op->nelem * prec_bytes(m->prec);
     h[k] = (char*)M.buf[MB_S] + off;
     h[nsys + k] = (char*)M.buf[MB_P0] + off;
-    h[3 * nsys + k] = (char*)M.buf[MB_R] + off;
-    for (int j = 1; j < nshift; ++j) h[4 * nsys + nsys * kMsShifts + k * kMsShifts + j] = m->p[k][j];
+    for (int q = 0; q < NR; ++q) h[3 * nsys + k * NR + q] = m->ring[k][q];
+    for (int j = 1; j < nshift; ++j)
+      h[3 * nsys + NR * nsys + nsys * kMsShifts + k * kMsShifts + j] = m->p[k][j];
     hn[k] = M.op->nelem;
   }
   if (hipMemcpy(m->dtab, h.data(), ptab, hipMemcpyHostToDevice) != hipSuccess ||
@@ -2471,7 +2483,7 @@ int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* 
     if (!(bn > 0)) return fail(DWF_ERR_INVALID_ARG, "zero source");
     const MsMember& M = m->mem[m->slot[k].first];
     const size_t off = (size_t)m->slot[k].second * op->nelem * eb;
-    HIPCHK(launch_copy(m->prec, (char*)M.buf[MB_R] + off, b[k], op->nelem, st));
+    HIPCHK(launch_copy(m->prec, m->ring[k][0], b[k], op->nelem, st));
     HIPCHK(launch_copy(m->prec, (char*)M.buf[MB_P0] + off, b[k], op->nelem, st));
     for (int j = 0; j < nshift; ++j) {
       if (j > 0) HIPCHK(launch_copy(m->prec, m->p[k][j], b[k], op->nelem, st));
@@ -2489,9 +2501,12 @@ int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* 
     S.nshift2 = nshift;
     S.nconv = -1;
     S.max_iter = max_iter;
+    S.nfold = nshift;   // q0 = m = 0: the block starts at ring slot 0 (r = b)
     for (int j = 0; j < kMsShifts; ++j) {
       S.zeta1[j] = S.zeta2[j] = 1.0;
       S.csh2[j] = j < nshift ? sigma[j] - sigma[0] : 0.0;
+      S.ppe[j] = bn;
+      S.fc[0][j][0] = S.fc[1][j][0] = 1.0;
     }
   }
   // solution pointers of this solve

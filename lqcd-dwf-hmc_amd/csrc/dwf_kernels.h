@@ -86,26 +86,35 @@ struct GroupLaunch {
   size_t lds = 0;
   void* table = nullptr;   // device memory (owned)
 };
+// Krylov-block fold of the multishift CG: the shifts' x and p are updated once
+// per block of at most kFoldMax iterations from the block's residuals.
+constexpr int kFoldMax = 16;
+constexpr int kFoldCoef = 2 + 2 * kFoldMax;   // per shift: a, b, c_1..c_K, d_1..d_K
 // Batched multishift CG (dwf_batch.hip): device state of one system (one
 // right-hand side with its shifts), the scalars of AShiftsolver_CG
-// (ashiftsolver_CG-tmpl.h:216-286).
+// (ashiftsolver_CG-tmpl.h:216-286) and the block fold's bookkeeping.  The
+// residuals live in a ring of kFoldMax + 1 fields per system: q0 is the slot of
+// the residual at the start of the current block, m the iterations done in it,
+// rcur the slot of the newest residual.  fold != 0: this iteration ends a block
+// (start slot fq0, length fm, shifts 1..fnfold-1, coefficients fc[cbuf ^ 1]).
 constexpr int kMsShifts = 32;
 struct MsState {
-  double rr, rr_new, pa, alpha_p, beta_p, snorm, sigma0, stop;
+  double rr, rr_new, pa, alpha_p, beta_p, snorm, sigma0, stop, alpha;
   int active, upd, conv, nshift2, nconv, max_iter;
-  double csh2[kMsShifts], zeta1[kMsShifts], zeta2[kMsShifts], pp[kMsShifts];
-  double coef[3 * kMsShifts];   // per shift: x += c0 p, p = c1 p + c2 r
+  int q0, m, rcur, nfold, fold, fq0, fm, fnfold, cbuf, pad_;
+  double csh2[kMsShifts], zeta1[kMsShifts], zeta2[kMsShifts], pp[kMsShifts], ppe[kMsShifts];
+  double fc[2][kMsShifts][kFoldCoef];
 };
 // Device pointer tables of one batched solve (all in device memory):
-// s, p0, x0, r: [nsys]; x, p: [nsys][kMsShifts] (slot 0 unused); n: reals per
-// field of each system; partial: nsys * kMsShifts * nblk doubles; pp_res:
-// nsys * kMsShifts; status: coherent host memory (2 doubles).
+// s, p0, x0: [nsys]; ring: [nsys][kFoldMax + 1]; x, p: [nsys][kMsShifts] (slot 0
+// unused); n: reals per field of each system; partial: nsys * kMsShifts * nblk
+// doubles; pp_res: nsys * kMsShifts; status: coherent host memory (2 doubles).
 struct MsLaunch {
   int nsys = 0, nblk = 0;
   void** s = nullptr;
   void** p0 = nullptr;
   void** x0 = nullptr;
-  void** r = nullptr;
+  void** ring = nullptr;
   void** x = nullptr;
   void** p = nullptr;
   const size_t* n = nullptr;
@@ -237,8 +246,6 @@ hipError_t launch_shift_update(void* const* x, void* const* p, const void* r,
 // block's residuals R[0..m) (m <= kFoldMax) and per-shift coefficients
 // [a, b, c_1..c_K, d_1..d_K] (kFoldCoef doubles, K = kFoldMax) on the device:
 //   x_j += b p_j + sum_l d_l R_l;  p_j = a p_j + sum_l c_l R_l;  pp[j] = |p_j|^2
-constexpr int kFoldMax = 16;
-constexpr int kFoldCoef = 2 + 2 * kFoldMax;
 hipError_t launch_ms_fold(void* const* x, void* const* p, const void* const* R,
                           const double* coef, int nsh, int m, size_t n, double* partial,
                           double* pp, hipStream_t st);
