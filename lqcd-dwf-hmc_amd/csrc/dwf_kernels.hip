@@ -513,6 +513,15 @@ template <typename T, int OP> constexpr int k_waves() {
   return (OP == K_SITE && sizeof(T) == 2) ? 3 : KCfg<T>::WAVES;
 }
 
+// Sites per block of a compiled-Ls kernel: THREADS / (Ls / VEC), at Ls = 12 a
+// multiple of 16 (whole waves of 12-thread sites whose 12 spinor rows per
+// chirality fill 16-column matrix-core tiles).  The launchers use the same rule.
+template <typename T> __host__ __device__ constexpr int spb_of(int Ls, int vec) {
+  return Ls == 12 ? (KCfg<T>::THREADS / (12 / vec)) / 16 * 16 : KCfg<T>::THREADS / (Ls / vec);
+}
+// the chiral epilogue on the matrix cores (Ls x Ls in a 16 x 16 tile)
+constexpr bool mfma_ls(int Ls) { return Ls == 16 || Ls == 12; }
+
 // LSC > 0: Ls and the block shape (THREADS / (LSC / VEC) sites) are compile-time
 // constants, so every load and LDS access uses an immediate offset; LSC = 0:
 // runtime Ls.  Thread (lsite, q) owns s = q*VEC .. q*VEC + VEC - 1 of its site.
@@ -546,7 +555,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
   const C ua = upd ? (C)(unum[0] / uden[0]) : C(0);
   const int Q = Ls / VEC;   // threads per site
   static_assert(KCfg<T>::THREADS % 64 == 0 && KCfg<T>::THREADS <= 512, "block of 1-8 waves");
-  const int spb = LSC > 0 ? KCfg<T>::THREADS / (LSC / VEC) : spb_rt;
+  const int spb = LSC > 0 ? spb_of<T>(LSC, VEC) : spb_rt;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // region A: hop staging (links, neighbours), later the chiral epilogue buffers
   // chiral rows of one site are 12*Ls + 1 apart, so the sites of a wave read
@@ -556,7 +565,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
   constexpr bool PK = std::is_same<T, __half>::value && VEC == 2 && LSC == 16 && DWF_HOP_PK16;
   constexpr int LROW = PK ? kPkStride : 144;   // link words per site in LDS
   const int stageC = (OP == K_SITE) ? 0 : (LROW * spb + 8 * spb);
-  const int epiC = MAT ? (LSC == 16 ? spb * 12 * 17 : 2 * spb * SU) : 0;
+  const int epiC = MAT ? (mfma_ls(LSC) ? spb * 12 * 17 : 2 * spb * SU) : 0;
   const int regionA = stageC > epiC ? stageC : epiC;
   C* shA = reinterpret_cast<C*>(smem_raw);
   C* shL = shA;
@@ -748,17 +757,21 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
   }
 
   if (OP != K_SITE) __syncthreads();  // staging area is reused below
-  if constexpr (LSC == 16) {
-    // Matrix-core epilogue.  Per chirality the block's core is a 16 x (12 spb)
+  if constexpr (mfma_ls(LSC)) {
+    // Matrix-core epilogue.  Per chirality the block's core is an Ls x (12 spb)
     // matrix U[s][col], col = lsite*12 + k; O = E U on 16-column tiles.  One
     // chirality at a time goes through one LDS buffer (column stride 17, bank
-    // spread), which halves the LDS per block.
+    // spread), which halves the LDS per block.  Ls = 12: the 12 x 12 matrices
+    // in 16 x 16 tiles, A rows and columns beyond 12 zero, three k-steps of the
+    // 16x16x4 chains, output rows s >= 12 (lanes 48-63) not stored.
     using MM = Mma<C>;
     using v4 = typename MM::v4;
     constexpr int LP = 17;
-    constexpr int NW = KCfg<T>::THREADS / 64;      // waves per block
-    constexpr int NT = KCfg<T>::THREADS / (16 / VEC) * 12 / 16;   // 16-column tiles per block
+    constexpr int SPB = spb_of<T>(LSC, VEC);
+    constexpr int NW = SPB * (LSC / VEC) / 64;     // waves per block
+    constexpr int NT = SPB * 12 / 16;              // 16-column tiles per block
     constexpr int NTW = (NT + NW - 1) / NW;        // tiles per wave
+    constexpr int LL = LSC * LSC;                  // one chirality's matrix
     C* mb = shA;
     const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -777,8 +790,9 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
         h4_t av, bv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          av[i] = (_Float16)M[(4 * lk + i) * 16 + arow];
-          bv[i] = (_Float16)mb[col * LP + 4 * lk + i];
+          const int sin = 4 * lk + i;
+          av[i] = (_Float16)(sin < LSC && arow < LSC ? M[sin * LSC + arow] : C(0));
+          bv[i] = (_Float16)(sin < LSC ? mb[col * LP + sin] : C(0));
         }
         return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, init, 0, 0, 0);
       } else if constexpr (std::is_same<T, __hip_bfloat16>::value && DWF_EPI_MFMA16) {
@@ -791,16 +805,17 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
         s4_t av, bv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          av[i] = bf(M[(4 * lk + i) * 16 + arow]);
-          bv[i] = bf(mb[col * LP + 4 * lk + i]);
+          const int sin = 4 * lk + i;
+          av[i] = bf(sin < LSC && arow < LSC ? M[sin * LSC + arow] : C(0));
+          bv[i] = bf(sin < LSC ? mb[col * LP + sin] : C(0));
         }
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, init, 0, 0, 0);
       } else {
         v4 a = init;
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) {
+        for (int kc = 0; kc < LSC / 4; ++kc) {
           const int sin = kc * 4 + lk;
-          a = MM::run(M[sin * 16 + arow], mb[col * LP + sin], a);
+          a = MM::run(arow < LSC ? M[sin * LSC + arow] : C(0), mb[col * LP + sin], a);
         }
         return a;
       }
@@ -829,7 +844,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
     __syncthreads();
     stage_acc(C(-1));
     __syncthreads();
-    mma_one(sh_E + 256, cl, false);
+    mma_one(sh_E + LL, cl, false);
     if (xmat) {   // + G X, X staged the same way
       auto stage_x = [&](C sign) {
         if (valid) {
@@ -850,7 +865,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
       __syncthreads();
       stage_x(C(-1));
       __syncthreads();
-      mma_one(sh_G + 256, cl, true);
+      mma_one(sh_G + LL, cl, true);
     }
     // combine, add X, store; lane: column col, rows s_out = 4 lk + r.  With
     // OUT2 the tile's chiral combinations are staged in its own LDS columns and
@@ -863,7 +878,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
       if (t < ntiles) {
         const int col = t * 16 + li;
         const int cs = col / 12, ck = col - cs * 12;
-        const bool cvalid = site0 + cs < VhV;
+        const bool cvalid = site0 + cs < VhV && 4 * lk < LSC;   // rows s >= Ls: padding
         const size_t ob = vaddr(site0 + cs) + (size_t)ck * Ls + 4 * lk;
         C o1[4], o2[4];   // rows k and k + 12
 #pragma unroll
@@ -934,7 +949,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
 #pragma unroll
           for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] - o2[r];
           wave_lds_sync();
-          const v4 fl = mma_tile(sh_F + 256, col, v4{0, 0, 0, 0});
+          const v4 fl = mma_tile(sh_F + LL, col, v4{0, 0, 0, 0});
           if (cvalid) {
             C p1[4], p2[4];
 #pragma unroll
@@ -1283,7 +1298,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   if (e.nrm_partial && !mat) return hipErrorInvalidValue;
   const int vec = KCfg<T>::VEC;
   const bool lsc = (Ls == 4 || Ls == 8 || Ls == 12 || Ls == 16) && Ls % vec == 0;
-  int spb = lsc ? KCfg<T>::THREADS / (Ls / vec) : KCfg<T>::THREADS / Ls;
+  int spb = lsc ? spb_of<T>(Ls, vec) : KCfg<T>::THREADS / Ls;
   if (spb < 1) return hipErrorInvalidValue;
   const size_t cs = sizeof(C);
   if ((e.G && (!mat || !addx)) || (e.X2 && !out2)) return hipErrorInvalidValue;
@@ -1294,7 +1309,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
     const bool pk = std::is_same<T, __half>::value && KCfg<T>::VEC == 2 && Ls == 16 &&
                     DWF_HOP_PK16;
     const size_t stage = (op == K_SITE) ? 0 : (size_t)((pk ? kPkStride : 144) + 8) * sp * cs;
-    const size_t epi = mat ? (size_t)sp * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
+    const size_t epi = mat ? (size_t)sp * (mfma_ls(Ls) && lsc ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;
     return (stage > epi ? stage : epi) + matb;
   };
   // The compiled-Ls kernels fix their block at THREADS / (Ls / VEC) sites; when
@@ -1302,7 +1317,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   // runtime-Ls kernel below runs instead, with fewer sites per block.
   // the fused CG updates, and an output that is not stored, exist in the
   // matrix-core epilogue only
-  const bool mfma_path = mat && Ls == 16 && lsc && lds_for(spb) <= lds_cap;
+  const bool mfma_path = mat && mfma_ls(Ls) && lsc && lds_for(spb) <= lds_cap;
   if (e.upd && !(mfma_path && (op == K_SITE ? addx : !out2))) return hipErrorInvalidValue;
   if (!out && !e.upd && !(mfma_path && out2)) return hipErrorInvalidValue;
   const int VhV = Vh << e.nrs;   // virtual sites (site, rhs)
@@ -1376,7 +1391,7 @@ template <typename T>
 static hipError_t prepare_group_t(int op, int Ls, int n, const GroupMember* m, GroupLaunch* gl) {
   using C = typename St<T>::C;
   if (KCfg<T>::VEC != 1 && Ls % KCfg<T>::VEC) return hipErrorInvalidValue;
-  const int spb = KCfg<T>::THREADS / (Ls / KCfg<T>::VEC);
+  const int spb = spb_of<T>(Ls, KCfg<T>::VEC);
   const Epi& e0 = m[0].e;
   const bool mat = e0.E != nullptr, addx = e0.X != nullptr, out2 = e0.out2 != nullptr;
   std::vector<DwfArgs<T>> tab(n);
@@ -1418,7 +1433,7 @@ static hipError_t prepare_group_t(int op, int Ls, int n, const GroupMember* m, G
   const size_t cs = sizeof(C);
   const size_t matb = 2 * (size_t)Ls * Ls * cs * (1 + (out2 ? 1 : 0) + (e0.G ? 1 : 0));
   const size_t stage = (op == K_SITE) ? 0 : (size_t)(144 + 8) * spb * cs;
-  const size_t epi = (size_t)spb * (Ls == 16 ? 12 * 17 : 2 * (12 * Ls + 1)) * cs;
+  const size_t epi = (size_t)spb * (mfma_ls(Ls) ? 12 * 17 : 2 * (12 * Ls + 1)) * cs;
   gl->lds = (stage > epi ? stage : epi) + matb;
   if (gl->lds > 64 * 1024) return hipErrorInvalidValue;
   gl->op = op;
