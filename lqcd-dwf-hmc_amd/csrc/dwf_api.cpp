@@ -1257,7 +1257,7 @@ struct FastCG {
   int init(dwf_solver_s* s) {
     op = s->op;
     pre = !op->mp_scalar;
-    fused = op->Ls == 16;
+    fused = op->Ls == 16 || op->Ls == 12;   // the matrix-core epilogue carries the updates
     for (int i = 0; i < (pre ? 5 : 4); ++i) RC(solver_field(op, &s->f[i]));
     x = s->f[0]; r = s->f[1]; p = s->f[2]; q = s->f[3]; p2 = s->f[4];
     res = g_ctx.red.result;

@@ -209,9 +209,9 @@ def test_richardson_mixed_precision_matches_oracle():
 
 
 @pytest.mark.parametrize("inner", ["half", "float", "bfloat16", "double"])
-@pytest.mark.parametrize("prm", LS16, ids=["shamir", "mobius"])
-def test_richardson_low_precision_inner_ls16(prm, inner):
-    """Mixed solve with the sync-free inner CG at Ls = 16 (matrix-core
+@pytest.mark.parametrize("prm", LS16 + LS12, ids=["ls16-shamir", "ls16-mobius", "ls12-mobius"])
+def test_richardson_low_precision_inner_matrix_core(prm, inner):
+    """Mixed solve with the sync-free inner CG at Ls = 16 and 12 (matrix-core
     epilogue, fused p update + Mp prologue): the FP64 residual reaches the
     stopping condition and the solution is the FP64 CG one."""
     orc, gpu, U = pair(prm)
