@@ -455,6 +455,7 @@ def main():
         log(f"HMC trajectory 32x16x16x24 Ls=12: {o['seconds']:.2f} s, H(diff) = "
             f"{o['H_diff']:.8f} (reference {ref_dh}, {ref_s} s on 4 GPUs)")
 
+
     cpu = None
     if not a.no_cpu_baseline:
         Us = sm.get_config(NSMEAR)

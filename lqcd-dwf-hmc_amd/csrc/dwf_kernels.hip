@@ -277,10 +277,10 @@ __device__ __forceinline__ float dot2_first(f16x2_t a, f16x2_t b) {
   return r;
 }
 
-template <int DIR, bool DAG>
+template <int DIR, bool DAG, int LSC>
 __device__ __forceinline__ void pk_load(f16x2_t (&w)[24], const __half* __restrict__ in,
                                         const int* shN, int lsite, int s0, int spb) {
-  constexpr int Ls = 16;
+  constexpr int Ls = LSC;
   const int nb = shN[DIR * spb + lsite];
   const __half* wp = in + (size_t)nb * 24 * Ls + s0;
   if (!DAG) {
@@ -345,19 +345,19 @@ __device__ __forceinline__ void pk_compute(float (*acc)[24], const f16x2_t (&w)[
     reconstruct<DIR>(acc[v], a1, a2);
   }
 }
-template <bool DAG>
+template <bool DAG, int LSC>
 __device__ __forceinline__ void hop_all_pk(float (*acc)[24], const __half* __restrict__ in,
                                            const unsigned* shP, const int* shN, int lsite,
                                            int s0, int spb) {
   f16x2_t wa[24];
-  pk_load<0, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<0>(acc, wa, shP, lsite);
-  pk_load<1, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<1>(acc, wa, shP, lsite);
-  pk_load<2, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<2>(acc, wa, shP, lsite);
-  pk_load<3, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<3>(acc, wa, shP, lsite);
-  pk_load<4, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<4>(acc, wa, shP, lsite);
-  pk_load<5, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<5>(acc, wa, shP, lsite);
-  pk_load<6, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<6>(acc, wa, shP, lsite);
-  pk_load<7, DAG>(wa, in, shN, lsite, s0, spb); pk_compute<7>(acc, wa, shP, lsite);
+  pk_load<0, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<0>(acc, wa, shP, lsite);
+  pk_load<1, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<1>(acc, wa, shP, lsite);
+  pk_load<2, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<2>(acc, wa, shP, lsite);
+  pk_load<3, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<3>(acc, wa, shP, lsite);
+  pk_load<4, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<4>(acc, wa, shP, lsite);
+  pk_load<5, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<5>(acc, wa, shP, lsite);
+  pk_load<6, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<6>(acc, wa, shP, lsite);
+  pk_load<7, DAG, LSC>(wa, in, shN, lsite, s0, spb); pk_compute<7>(acc, wa, shP, lsite);
 }
 
 template <bool DAG, typename T, int VEC, typename C>
@@ -561,8 +561,8 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
   // chiral rows of one site are 12*Ls + 1 apart, so the sites of a wave read
   // their rows from different LDS banks
   const int SU = 12 * Ls + 1;
-  // packed FP16 hop (two s per thread, Ls = 16)
-  constexpr bool PK = std::is_same<T, __half>::value && VEC == 2 && LSC == 16 && DWF_HOP_PK16;
+  // packed FP16 hop (two s per thread, Ls = 16 or 12)
+  constexpr bool PK = std::is_same<T, __half>::value && VEC == 2 && mfma_ls(LSC) && DWF_HOP_PK16;
   constexpr int LROW = PK ? kPkStride : 144;   // link words per site in LDS
   const int stageC = (OP == K_SITE) ? 0 : (LROW * spb + 8 * spb);
   const int epiC = MAT ? (mfma_ls(LSC) ? spb * 12 * 17 : 2 * spb * SU) : 0;
@@ -717,13 +717,13 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
       }
     } else if (OP == K_HOP) {
       if constexpr (PK)
-        hop_all_pk<false>(acc, reinterpret_cast<const __half*>(inr),
+        hop_all_pk<false, LSC>(acc, reinterpret_cast<const __half*>(inr),
                           reinterpret_cast<const unsigned*>(shA), shN, rl, s0, spb);
       else
         hop_all<false, T, VEC>(acc, inr, shL, shN, rl, s0, spb, Ls);
     } else {
       if constexpr (PK)
-        hop_all_pk<true>(acc, reinterpret_cast<const __half*>(inr),
+        hop_all_pk<true, LSC>(acc, reinterpret_cast<const __half*>(inr),
                          reinterpret_cast<const unsigned*>(shA), shN, rl, s0, spb);
       else
         hop_all<true, T, VEC>(acc, inr, shL, shN, rl, s0, spb, Ls);
@@ -1306,7 +1306,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
       mat ? 2 * (size_t)Ls * Ls * cs * (1 + (out2 ? 1 : 0) + (e.G ? 1 : 0)) : 0;
   const size_t lds_cap = 64 * 1024;
   auto lds_for = [&](int sp) {
-    const bool pk = std::is_same<T, __half>::value && KCfg<T>::VEC == 2 && Ls == 16 &&
+    const bool pk = std::is_same<T, __half>::value && KCfg<T>::VEC == 2 && mfma_ls(Ls) &&
                     DWF_HOP_PK16;
     const size_t stage = (op == K_SITE) ? 0 : (size_t)((pk ? kPkStride : 144) + 8) * sp * cs;
     const size_t epi = mat ? (size_t)sp * (mfma_ls(Ls) && lsc ? 12 * 17 : 2 * (12 * Ls + 1)) * cs : 0;

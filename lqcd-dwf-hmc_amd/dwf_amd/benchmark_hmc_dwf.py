@@ -22,16 +22,19 @@ GOLDEN = os.path.join(os.path.dirname(PKG), "tests", "golden")
 
 
 def run_trajectory(lattice_size, conf_npy=None, rational_dir=None, i_conf=200000,
-                   log=None):
+                   log=None, inner_precision="float", inner_stop=1e-8):
     """One trajectory of the benchmark; returns the HMC_General.update result
-    with "seconds" (wall time of the trajectory, links already on the GPU)."""
+    with "seconds" (wall time of the trajectory, links already on the GPU).
+    inner_precision / inner_stop: the Richardson solves' inner CG ("float" and
+    1e-8 as in the reference)."""
     import numpy as np
     import dwf_amd
     from dwf_amd import hmc, lattice
     conf_npy = conf_npy or os.path.join(GOLDEN, "conf_04040408.npy")
     rational_dir = rational_dir or os.path.join(GOLDEN, "rational")
     dims = tuple(int(v) for v in lattice_size)
-    hmcobj, rng = hmc.benchmark_nf2p1(dims, rational_dir)
+    hmcobj, rng = hmc.benchmark_nf2p1(dims, rational_dir, inner_precision=inner_precision,
+                                      inner_stop=inner_stop)
     U = hmc.GaugeField(dims)
     U.from_host(lattice.tile_4x4x4x8(np.load(conf_npy), dims))
     rng.reset(i_conf)
@@ -60,10 +63,15 @@ def main():
     ap.add_argument("--lattice", type=int, nargs=4, default=[8, 8, 8, 8])
     ap.add_argument("--conf", default=None, help="site-major (V, 4, 18) .npy of the 4^3x8 field")
     ap.add_argument("--rational", default=None, help="folder of parameter_rational.* files")
+    ap.add_argument("--inner", default="float", choices=["float", "half"],
+                    help="storage of the Richardson solves' inner CG")
+    ap.add_argument("--inner-stop", type=float, default=None,
+                    help="inner |r|^2/|b|^2 (default 1e-8 for float, 2e-4 for half)")
     a = ap.parse_args()
     sys.path.insert(0, PKG)
-    run_trajectory(a.lattice, a.conf, a.rational,
-                   log=lambda s: print(s, flush=True))
+    stop = a.inner_stop if a.inner_stop is not None else (1e-8 if a.inner == "float" else 2e-4)
+    run_trajectory(a.lattice, a.conf, a.rational, log=lambda s: print(s, flush=True),
+                   inner_precision=a.inner, inner_stop=stop)
 
 
 if __name__ == "__main__":

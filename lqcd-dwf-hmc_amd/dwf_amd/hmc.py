@@ -170,12 +170,15 @@ class Fprop_alt_Standard_eo_Richardson:
     """Fprop_alt_Standard_eo_Richardson in mode DdagD_even
     (src/lib_alt/Measurements/Fermion/fprop_alt_Standard_eo_Richardson-tmpl.h:27-102,
     460-481): FP64 Richardson to convergence_criterion_squared around an FP32
-    CG to 1e-8 with max_iter * max_restart iterations."""
+    CG to 1e-8 with max_iter * max_restart iterations (the reference's scheme;
+    fopr_f may be an FP16-storage operator with a looser inner_stop)."""
 
-    def __init__(self, fopr_d, fopr_f, max_iter=100, max_restart=40, stop_cond=1e-28):
+    def __init__(self, fopr_d, fopr_f, max_iter=100, max_restart=40, stop_cond=1e-28,
+                 inner_stop=1e-8):
         self.fopr_d, self.fopr_f = fopr_d, fopr_f
         self.max_iter, self.inner_max = int(max_iter), int(max_iter * max_restart)
         self.stop = float(stop_cond)
+        self.inner_stop = float(inner_stop)
         self._solver = None
         self.nconv = self.diff = None
 
@@ -184,7 +187,8 @@ class Fprop_alt_Standard_eo_Richardson:
         d.set_mode("DdagD")
         f.set_mode("DdagD")
         if self._solver is None:
-            self._solver = ASolver_Richardson(d, f, self.max_iter, self.stop, self.inner_max, 1e-8)
+            self._solver = ASolver_Richardson(d, f, self.max_iter, self.stop, self.inner_max,
+                                              self.inner_stop)
         self.nconv, self.diff = self._solver.solve(x, b)
         return self.nconv, self.diff
 
@@ -516,13 +520,17 @@ def benchmark_nf2p1(lattice_size, rational_dir, seed=1234567, Nsteps=(2, 1, 3),
                     masses=(0.02, 0.0005, 0.002, 1.0), beta=4.17,
                     c_plaq=1.66666666666666666666, c_rect=-0.08333333333333333333,
                     rho=0.1, nsmear=3, npoles_MD=16, npoles_H=24, lower=1.0e-2,
-                    upper=3.16227766, stop_MD=1e-26, stop_H=1e-28, stop_solver=1e-28):
+                    upper=3.16227766, stop_MD=1e-26, stop_H=1e-28, stop_solver=1e-28,
+                    inner_precision="float", inner_stop=1e-8):
     """The 2+1-flavour domain-wall HMC of the reference benchmark
     (benchmark_hmc_dwf/hmc_dwf_eo.cpp:180-360 with HMC_DWF_Nf2p1_eo.yaml): a
     one-flavour rational ratio (Pauli-Villars / Nf1) and two Hasenbusch ratios
     (Nf2_prec / Nf2 and PV / Nf2_prec) on 3-level stout links, the
     plaquette + rectangle gauge action, Omelyan on 3 levels.  masses: (Nf1,
-    Nf2, Nf2_prec, Pauli-Villars).  Returns (hmc, rng)."""
+    Nf2, Nf2_prec, Pauli-Villars).  inner_precision / inner_stop: the inner CG of
+    the Richardson solves ("float" and 1e-8 as in the reference; "half" storage
+    with a looser criterion runs more outer steps, each cheaper, to the same
+    FP64 stopping condition).  Returns (hmc, rng)."""
     dims = tuple(int(v) for v in lattice_size)
     director = Director(dims, rho, nsmear)
     m_nf1, m_nf2, m_prec, m_pv = masses
@@ -531,10 +539,11 @@ def benchmark_nf2p1(lattice_size, rational_dir, seed=1234567, Nsteps=(2, 1, 3),
         return SmearedFopr(director, dims, Ls, mq, M0, b, c, (1, 1, 1, -1), precision)
 
     pv, nf1, nf2, prec = fopr(m_pv), fopr(m_nf1), fopr(m_nf2), fopr(m_prec)
-    pv_f, nf2_f, prec_f = fopr(m_pv, "float"), fopr(m_nf2, "float"), fopr(m_prec, "float")
+    pv_f, nf2_f, prec_f = (fopr(m_pv, inner_precision), fopr(m_nf2, inner_precision),
+                           fopr(m_prec, inner_precision))
 
     def fprop(d, f):
-        return Fprop_alt_Standard_eo_Richardson(d, f, 100, 40, stop_solver)
+        return Fprop_alt_Standard_eo_Richardson(d, f, 100, 40, stop_solver, inner_stop)
 
     r = {
         "num_MD": rational(rational_dir, 1, 4, npoles_MD, lower, upper, 1000, stop_MD),

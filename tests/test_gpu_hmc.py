@@ -211,3 +211,20 @@ def test_reference_trajectory_32x16x16x24():
     assert abs(out["plaq_init"] - KNOWN["plaq_init" + sfx]["value"]) < 1e-12
     assert abs(out["plaq_trial"] - KNOWN["plaq_trial" + sfx]["value"]) < 1e-11
     assert abs(out["H_diff"] - KNOWN["H_diff" + sfx]["value"]) < 1e-6
+
+
+def test_reference_trajectory_fp16_inner():
+    """The 8^4 reference trajectory with the Richardson solves' inner CG on
+    FP16-storage operators (inner |r|^2/|b|^2 < 2e-4, more outer steps to the
+    same FP64 stopping condition 1e-28): the energies and H(diff) are those of
+    the reference to 1e-6, as with its FP32 inner CG."""
+    hmcobj, rng = hmc.benchmark_nf2p1(DIMS8, RATIONAL, inner_precision="half", inner_stop=2e-4)
+    rng.reset(200000)
+    out = hmcobj.update(dev(conf8(), DIMS8))
+    print(f"  FP16 inner: H_diff {out['H_diff']:.10f}", flush=True)
+    pairs = [("H_kin", "H_kin_trial"), ("H_0_Frational", "H_Frational_trial"),
+             ("H_1_Fratio_Nf2", "H_Fratio_Nf2_trial"), ("H_2_Fratio_Nf2prec", "H_Fratio_Nf2prec_trial"),
+             ("H_3_gauge", "H_Grectangle_trial")]
+    for mine, ref in pairs:
+        assert abs(out["trial"][mine] - KNOWN[ref]["value"]) < 1e-6, (mine, out["trial"][mine])
+    assert abs(out["H_diff"] - KNOWN["H_diff"]["value"]) < 1e-6
