@@ -22,11 +22,12 @@ GOLDEN = os.path.join(os.path.dirname(PKG), "tests", "golden")
 
 
 def run_trajectory(lattice_size, conf_npy=None, rational_dir=None, i_conf=200000,
-                   log=None, inner_precision="float", inner_stop=1e-8):
+                   log=None, inner_precision="float", inner_stop=1e-8, **setup):
     """One trajectory of the benchmark; returns the HMC_General.update result
     with "seconds" (wall time of the trajectory, links already on the GPU).
     inner_precision / inner_stop: the Richardson solves' inner CG ("float" and
-    1e-8 as in the reference)."""
+    1e-8 as in the reference); setup: other benchmark_nf2p1 parameters (Ls,
+    masses, Nsteps, trajectory_length, ...)."""
     import numpy as np
     import dwf_amd
     from dwf_amd import hmc, lattice
@@ -34,7 +35,7 @@ def run_trajectory(lattice_size, conf_npy=None, rational_dir=None, i_conf=200000
     rational_dir = rational_dir or os.path.join(GOLDEN, "rational")
     dims = tuple(int(v) for v in lattice_size)
     hmcobj, rng = hmc.benchmark_nf2p1(dims, rational_dir, inner_precision=inner_precision,
-                                      inner_stop=inner_stop)
+                                      inner_stop=inner_stop, **setup)
     U = hmc.GaugeField(dims)
     U.from_host(lattice.tile_4x4x4x8(np.load(conf_npy), dims))
     rng.reset(i_conf)
@@ -63,15 +64,23 @@ def main():
     ap.add_argument("--lattice", type=int, nargs=4, default=[8, 8, 8, 8])
     ap.add_argument("--conf", default=None, help="site-major (V, 4, 18) .npy of the 4^3x8 field")
     ap.add_argument("--rational", default=None, help="folder of parameter_rational.* files")
-    ap.add_argument("--inner", default="float", choices=["float", "half"],
+    ap.add_argument("--inner", default="float", choices=["double", "float", "half"],
                     help="storage of the Richardson solves' inner CG")
     ap.add_argument("--inner-stop", type=float, default=None,
-                    help="inner |r|^2/|b|^2 (default 1e-8 for float, 2e-4 for half)")
+                    help="inner |r|^2/|b|^2 (default 1e-8 for float and double, 2e-4 for half)")
+    ap.add_argument("--Ls", type=int, default=12)
+    ap.add_argument("--masses", type=float, nargs=4, default=None,
+                    help="Nf1, Nf2, Nf2_prec, Pauli-Villars (default 0.02 0.0005 0.002 1.0)")
+    ap.add_argument("--steps", type=int, nargs=3, default=[2, 1, 3])
+    ap.add_argument("--tau", type=float, default=0.1)
     a = ap.parse_args()
     sys.path.insert(0, PKG)
-    stop = a.inner_stop if a.inner_stop is not None else (1e-8 if a.inner == "float" else 2e-4)
+    stop = a.inner_stop if a.inner_stop is not None else (2e-4 if a.inner == "half" else 1e-8)
+    setup = dict(Ls=a.Ls, Nsteps=tuple(a.steps), trajectory_length=a.tau)
+    if a.masses:
+        setup["masses"] = tuple(a.masses)
     run_trajectory(a.lattice, a.conf, a.rational, log=lambda s: print(s, flush=True),
-                   inner_precision=a.inner, inner_stop=stop)
+                   inner_precision=a.inner, inner_stop=stop, **setup)
 
 
 if __name__ == "__main__":

@@ -228,3 +228,23 @@ def test_reference_trajectory_fp16_inner():
     for mine, ref in pairs:
         assert abs(out["trial"][mine] - KNOWN[ref]["value"]) < 1e-6, (mine, out["trial"][mine])
     assert abs(out["H_diff"] - KNOWN["H_diff"]["value"]) < 1e-6
+
+
+def test_config5_parameters_mixed_matches_fp64():
+    """The 2+1-flavour trajectory with the light mass 0.01 and Ls = 16 (the
+    24^3 x 48 workload's parameters, here on 16^3 x 32): the Richardson solves'
+    inner CG in FP16 storage (2e-4) and in FP64 (1e-8) stop on the same FP64
+    residual, so H(diff) agrees to 1e-6 (measured at 24^3 x 48: 0.19950747 and
+    0.19950745)."""
+    dims = (16, 16, 16, 32)
+    U0 = lattice.tile_4x4x4x8(np.load(os.path.join(HERE, "golden", "conf_04040408.npy")), dims)
+    res = {}
+    for inner, stop in (("half", 2e-4), ("double", 1e-8)):
+        h, rng = hmc.benchmark_nf2p1(dims, RATIONAL, Ls=16, masses=(0.04, 0.01, 0.1, 1.0),
+                                     inner_precision=inner, inner_stop=stop)
+        rng.reset(200000)
+        res[inner] = h.update(dev(U0, dims))
+        print(f"  {inner} inner: H_diff {res[inner]['H_diff']:.10f}", flush=True)
+    assert abs(res["half"]["H_diff"] - res["double"]["H_diff"]) < 1e-6
+    for k, v in res["double"]["trial"].items():
+        assert abs(res["half"]["trial"][k] - v) < 1e-6 * max(1.0, abs(v) * 1e-8), k
