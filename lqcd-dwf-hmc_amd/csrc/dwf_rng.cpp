@@ -12,15 +12,18 @@
 // src/lib/Tools/randomNumbers.cpp:64-76).
 // Parallel streams: one generator per (z, t) slice, seeded (seed + 1) Lz Lt + z
 // + Lz t and advanced 1000 draws, filling the lexical sites of its slice in x,
-// y order, the extra index outermost (RandomNumbers_parallel::init_2d /
+// y order, the extra index outermost (the streams run on host threads)
+// (RandomNumbers_parallel::init_2d /
 // set_lex_global_2d, src/lib/Tools/randomNumbers_parallel-tmpl.h:146-230,
 // randomNumbers_parallel_Mseries.cpp:29-38).  The numbers must be
 // bit-identical to the reference's for the HMC energies to agree, so the
 // arithmetic follows those steps exactly.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dwf_hip.h"
@@ -115,6 +118,22 @@ class Mseries {
 const double Mseries::kSq2r = 1.0 / std::sqrt(2.0);
 const double Mseries::kPi2 = 2.0 * (4.0 * std::atan(1.0));
 
+// f(i) for i in [0, n) on up to 16 host threads (the (z, t) streams are
+// independent, so each thread runs whole streams in their own order)
+template <typename F> void parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+  if (nt == 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t]() {
+      for (int i = t; i < n; i += nt) f(i);
+    });
+  for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 struct dwf_rng_s {
@@ -125,14 +144,14 @@ struct dwf_rng_s {
   void init(unsigned long seed) {
     const int Nz = N[2], Nt = N[3];
     master.reset(new Mseries((int)seed));
+    slice.clear();
     slice.resize((size_t)Nz * Nt);
-    for (int t = 0; t < Nt; ++t)
-      for (int z = 0; z < Nz; ++z) {
-        const unsigned long s = (seed + 1) * (unsigned long)Nz * Nt + (unsigned long)(z + Nz * t);
-        auto& g = slice[z + Nz * t];
-        g.reset(new Mseries((int)s));
-        for (int i = 0; i < 1000; ++i) g->get();
-      }
+    parallel_for(Nz * Nt, [&](int zt) {
+      const unsigned long s = (seed + 1) * (unsigned long)Nz * Nt + (unsigned long)zt;
+      auto& g = slice[zt];
+      g.reset(new Mseries((int)s));
+      for (int i = 0; i < 1000; ++i) g->get();
+    });
   }
 };
 
@@ -174,16 +193,17 @@ int dwf_rng_gauss_lex_global(dwf_rng r, int nin, int nex, double* out) {
   if (!r || !out || nin < 2 || (nin & 1) || nex < 1) return DWF_ERR_INVALID_ARG;
   const int Nx = r->N[0], Ny = r->N[1], Nz = r->N[2], Nt = r->N[3];
   const size_t V = (size_t)Nx * Ny * Nz * Nt;
-  for (int j = 0; j < nex; ++j)
-    for (int t = 0; t < Nt; ++t)
-      for (int z = 0; z < Nz; ++z) {
-        Mseries& g = *r->slice[z + Nz * t];
-        size_t site = (size_t)Nx * Ny * (z + (size_t)Nz * t);
-        for (int yx = 0; yx < Nx * Ny; ++yx, ++site) {
-          double* o = out + (size_t)nin * (site + V * j);
-          for (int i = 0; i < nin; i += 2) g.gauss(o[i], o[i + 1]);
-        }
+  // each (z, t) stream draws, in its own order, extra index j outermost
+  parallel_for(Nz * Nt, [&](int zt) {
+    Mseries& g = *r->slice[zt];
+    for (int j = 0; j < nex; ++j) {
+      size_t site = (size_t)Nx * Ny * zt;
+      for (int yx = 0; yx < Nx * Ny; ++yx, ++site) {
+        double* o = out + (size_t)nin * (site + V * j);
+        for (int i = 0; i < nin; i += 2) g.gauss(o[i], o[i + 1]);
       }
+    }
+  });
   return DWF_OK;
 }
 
