@@ -425,6 +425,11 @@ int dwf_rng_get(dwf_rng r, double* value);
 /* gauss_lex_global(Field(nin, V, nex)): Gaussian numbers of variance 1/2 into
  * the host array out[j][site][i] (lexical sites; nin even). */
 int dwf_rng_gauss_lex_global(dwf_rng r, int nin, int nex, double* out);
+/* The same numbers kept for the sites of one parity only, in the even-odd order
+ * site2 = x/2 + Nx/2*(y + Ny*(z + Nz*t)): out[j][site2][i] (gauss_lex_global then
+ * Index_eo::convertField(., ., parity), src/lib/Field/index_eo.cpp:125-156, as the
+ * pseudofermion noise of action_F_alt_Ratio_eo-tmpl.h langevin; Nx even). */
+int dwf_rng_gauss_eo_global(dwf_rng r, int nin, int nex, int parity, double* out);
 
 /* ------------------------------------------------------------------------
  * HMC gauge part, FP64, device arrays in the Field_G layout (4, V, 18).

@@ -207,4 +207,33 @@ int dwf_rng_gauss_lex_global(dwf_rng r, int nin, int nex, double* out) {
   return DWF_OK;
 }
 
+// The numbers of gauss_lex_global, stored only for the sites of one parity in
+// the even-odd order site2 = x/2 + Nx/2 (y + Ny (z + Nz t)) (gauss_lex_global
+// followed by Index_eo::convertField(., ., parity), src/lib/Field/index_eo.cpp:
+// 125-156): out[j][site2][i]; every site still draws its numbers.
+int dwf_rng_gauss_eo_global(dwf_rng r, int nin, int nex, int parity, double* out) {
+  if (!r || !out || nin < 2 || (nin & 1) || nex < 1 || (parity != 0 && parity != 1) ||
+      (r->N[0] & 1))
+    return DWF_ERR_INVALID_ARG;
+  const int Nx = r->N[0], Ny = r->N[1], Nz = r->N[2], Nt = r->N[3];
+  const size_t Vh = (size_t)Nx * Ny * Nz * Nt / 2;
+  parallel_for(Nz * Nt, [&](int zt) {
+    Mseries& g = *r->slice[zt];
+    const int z = zt % Nz, t = zt / Nz;
+    double skip[2];
+    for (int j = 0; j < nex; ++j)
+      for (int y = 0; y < Ny; ++y)
+        for (int x = 0; x < Nx; ++x) {
+          if (((x + y + z + t) & 1) == parity) {
+            const size_t site2 = x / 2 + (size_t)(Nx / 2) * (y + (size_t)Ny * zt);
+            double* o = out + (size_t)nin * (site2 + Vh * j);
+            for (int i = 0; i < nin; i += 2) g.gauss(o[i], o[i + 1]);
+          } else {
+            for (int i = 0; i < nin; i += 2) g.gauss(skip[0], skip[1]);
+          }
+        }
+  });
+  return DWF_OK;
+}
+
 }  // extern "C"

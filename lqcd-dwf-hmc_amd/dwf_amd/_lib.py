@@ -173,6 +173,7 @@ _SIGS = {
     "dwf_rng_destroy": (C.c_int, [_vp]),
     "dwf_rng_get": (C.c_int, [_vp, _dp]),
     "dwf_rng_gauss_lex_global": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "dwf_rng_gauss_eo_global": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "dwf_hmc_set_iP": (C.c_int, [_ip, _vp, _dp, _dp]),
     "dwf_hmc_exp_update": (C.c_int, [_ip, _vp, C.c_double, _vp, C.c_int]),
     "dwf_hmc_rect_action": (C.c_int, [_ip, _vp, C.c_double, C.c_double, C.c_double, _dp, _dp]),

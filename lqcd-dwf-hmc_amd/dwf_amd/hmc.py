@@ -59,6 +59,15 @@ class RandomNumbers_parallel_Mseries:
                    _lib.load().dwf_rng_gauss_lex_global(self._h, int(nin), int(nex), _ptr(out)))
         return out
 
+    def gauss_eo_global(self, nin, nex, parity=0):
+        """gauss_lex_global followed by Index_eo::convertField(., ., parity): the
+        numbers of the sites of one parity, (nex, V/2, nin) in even-odd order."""
+        Vh = int(np.prod(self.lattice_size)) // 2
+        out = np.empty((nex, Vh, nin))
+        _lib.check("dwf_rng_gauss_eo_global", _lib.load().dwf_rng_gauss_eo_global(
+            self._h, int(nin), int(nex), int(parity), _ptr(out)))
+        return out
+
     def __del__(self):
         if getattr(self, "_h", None):
             try:
@@ -229,7 +238,7 @@ class Action_G_Rectangle:
 def _noise_field(rng, fopr, Ls):
     """Gaussian pseudofermion noise: gauss_lex_global on Field(24, V, Ls), even
     sites, as a device field of fopr; returns (field, |xi|^2)."""
-    xi = even_part(rng.gauss_lex_global(24, Ls), fopr.lattice_size)
+    xi = rng.gauss_eo_global(24, Ls, 0)
     f = fopr.new_field()
     fopr.convert(f, xi)
     return f, float(np.sum(xi * xi))

@@ -55,11 +55,35 @@ def test_rng_matches_reference_on_32x16x16x24():
     del P
     assert abs(H_kin - KNOWN["H_kin_init_32x16x16x24"]["value"]) < 1e-6
     for key in ("H_Frational_init", "H_Fratio_Nf2_init", "H_Fratio_Nf2prec_init"):
-        xi = even_part(r.gauss_lex_global(24, 12), dims)
+        xi = r.gauss_eo_global(24, 12, 0)
         h = float(np.sum(xi * xi))
         want = KNOWN[key + "_32x16x16x24"]["value"]
         # H_F = |xi|^2 to the solver's precision (the reference prints |phi^+ M^-1 phi|)
         assert abs(h - want) / want < 1e-9, (key, h, want)
+
+
+def test_rng_even_odd_draws_equal_lexical_then_even_part():
+    """gauss_eo_global(nin, nex, p) = even_part-style selection of gauss_lex_global
+    (Index_eo::convertField) from the same stream positions, both parities, and
+    the streams end in the same state."""
+    dims = (8, 4, 4, 6)
+    a = RandomNumbers_parallel_Mseries(dims, 77)
+    b = RandomNumbers_parallel_Mseries(dims, 77)
+    for parity in (0, 1):
+        lex = a.gauss_lex_global(24, 3)
+        want = even_part(lex, dims) if parity == 0 else _odd_part(lex, dims)
+        got = b.gauss_eo_global(24, 3, parity)
+        assert np.array_equal(got, want)
+    assert np.array_equal(a.gauss_lex_global(8, 1), b.gauss_lex_global(8, 1))
+
+
+def _odd_part(xi_lex, dims):
+    Nx, Ny, Nz, Nt = dims
+    T, Z, Y, X2 = np.meshgrid(np.arange(Nt), np.arange(Nz), np.arange(Ny), np.arange(Nx // 2),
+                              indexing="ij")
+    x = 2 * X2 + (Y + Z + T + 1) % 2
+    lex = (x + Nx * (Y + Ny * (Z + Nz * T))).reshape(-1)
+    return xi_lex[:, lex, :]
 
 
 def test_rng_reset_restarts_the_streams():
