@@ -8,9 +8,10 @@ rho = 0.1 on the GPU; Gaussian complex source on the even sites; target
 |r|/|b| < 1e-10 on the FP64 true residual of DdagD x = b.
 
 One timed step = one solve by FP64 Richardson (the reference's
-Fprop_alt_Standard_eo_Richardson outer solver) around a sync-free CG on the
-FP16-storage operator with inner |r|^2/|b|^2 < 2e-4 (the fastest measured
-configuration).  Also measured, outside the timed steps: the plain FP64 CG
+Fprop_alt_Standard_eo_Richardson outer solver) around a sync-free CG on a
+reduced-precision operator; the inner storage type (FP16, FP32, BF16) and
+inner |r|^2/|b|^2 criterion (1e-4, 2e-4, 5e-4) are chosen before the timed
+steps by the library's schedule autotuner (one timed solve per candidate).  Also measured, outside the timed steps: the plain FP64 CG
 solve of the same system, the reference's scheme (FP32 inner CG to 1e-8), the
 Richardson with the learned polynomial preconditioner (training time apart),
 and the FP64 even-odd Dslash D_eo (one fused hop kernel) timed with HIP events
@@ -308,9 +309,25 @@ def main():
     log(f"operator error vs FP64: {op_err}")
 
     # ---- timed steps: mixed-precision solves --------------------------------
-    # FP64 Richardson (the reference's outer solver) around a sync-free CG on the
-    # FP16-storage operator, inner |r|^2/|b|^2 < INNER_TOL2_FAST.
-    rich = dwf_amd.ASolver_Richardson(f64, f16, 100, TOL2, 100000, INNER_TOL2_FAST)
+    # FP64 Richardson (the reference's outer solver) around a sync-free CG; the
+    # inner storage type and criterion come from the schedule autotuner (one timed
+    # solve per candidate, outside the timed steps)
+    fbf = dwf_amd.AFopr_Domainwall_5din_eo(DIMS, LS, MQ, M0, B, CC, (1, 1, 1, -1), "bfloat16")
+    fbf.set_config_smeared(sm, None)
+    fbf.set_mode("DdagD")
+    t_tune = time.perf_counter()
+    cand_ops, cand_names = [f16, f32, fbf], ["half", "float", "bfloat16"]
+    cand_tols = [1e-4, INNER_TOL2_FAST, 5e-4]
+    rich, tune_times, (ti, tj) = dwf_amd.tune_richardson(f64, cand_ops, cand_tols, b, x, 100,
+                                                         TOL2, 100000)
+    t_tune = time.perf_counter() - t_tune
+    tuned = {"chosen": {"inner_storage": cand_names[ti], "inner_criterion": cand_tols[tj]},
+             "tuning_s": round(t_tune, 2),
+             "candidate_solve_ms": {f"{cand_names[a]}/{cand_tols[c]:g}":
+                                    (round(float(tune_times[a, c]) * 1e3, 2) if tune_times[a, c] > 0 else None)
+                                    for a in range(len(cand_names)) for c in range(len(cand_tols))}}
+    log(f"schedule autotuner ({t_tune:.1f} s): {tuned['candidate_solve_ms']} -> "
+        f"{cand_names[ti]} inner, {cand_tols[tj]:g}")
     for i in range(a.warmup):
         n, d = rich.solve(x, b)
         log(f"warmup mixed solve: nconv={n} |r|/|b|={d:.3e} inner={rich.inner_iterations()}")
@@ -325,7 +342,7 @@ def main():
     for n, d, inner in results:
         if not (n >= 0 and d < 1e-10):
             raise RuntimeError(f"mixed solve missed the target: nconv={n} |r|/|b|={d}")
-    log(f"mixed FP64/FP16-storage solve: {t_mixed:.3f} s, outer={results[-1][0]}, "
+    log(f"mixed FP64/{cand_names[ti]}-storage solve: {t_mixed:.3f} s, outer={results[-1][0]}, "
         f"inner CG iterations={results[-1][2]}, |r|/|b|={results[-1][1]:.3e}")
 
     # ---- comparisons (not in the timed steps) --------------------------------
@@ -480,7 +497,8 @@ def main():
             "workload": "mixed-precision solve DdagD x = b to |r|/|b|<1e-10, 16^3x32, Ls=16, "
                         "mq=0.01, M0=1.6, Shamir, exp(0.3 iH) links + 2 stout(0.1) steps",
             "lattice": list(DIMS), "Ls": LS,
-            "step": "one FP64-Richardson / FP16-storage-CG solve (inner |r|^2/|b|^2 < 2e-4)",
+            "step": "one FP64-Richardson / low-precision-CG solve, inner storage and "
+                    "|r|^2/|b|^2 criterion chosen by the schedule autotuner",
             "value": "FP64 D_eo algorithmic HBM GB/s",
         },
         "time_to_solution_s": {
@@ -488,8 +506,8 @@ def main():
             "fp64_cg_iterations": fp64_iters,
             "fp64_cg_sync_free_fused": None if t_fp64f is None else round(t_fp64f, 4),
             "fp64_cg_sync_free_fused_iterations": i_fp64f,
-            "mixed_fp64_fp16_storage": round(t_mixed, 4),
-            "mixed_fp64_fp16_outer_inner": [results[-1][0], results[-1][2]],
+            "mixed_fp64_tuned_inner": round(t_mixed, 4),
+            "mixed_fp64_tuned_inner_outer_inner": [results[-1][0], results[-1][2]],
             "reference_scheme_fp64_fp32_inner_1e-8": round(t_ref, 4),
             "mixed_learned": round(t_lpc, 4),
             "learned_training": round(t_train, 4),
@@ -501,6 +519,7 @@ def main():
             "speedup_learned_vs_fp64_incl_training":
                 None if t_fp64 is None else round(t_fp64 / (t_lpc + t_train), 3),
         },
+        "schedule_autotuner": tuned,
         "multishift_16_shifts_fp64": {"time_s": round(t_ms, 4), "iterations": n_ms,
                                       "ms_per_iteration": round(t_ms / max(n_ms, 1) * 1e3, 3),
                                       "max_rel_residual": d_ms},

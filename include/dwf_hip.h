@@ -163,6 +163,17 @@ int dwf_solver_create_cg(dwf_fopr op, int max_iter, double stop_cond,
 int dwf_solver_create_richardson(dwf_fopr op_d, dwf_fopr op_f, int max_iter,
                                  double stop_cond, int max_iter_inner,
                                  double stop_cond_inner, dwf_solver* out);
+/* Schedule autotuner (no counterpart in the reference; for callers that choose
+ * the mixed-precision scheme per lattice): for each inner operator inner[i]
+ * (same lattice and parameters as op_d, any storage type) and inner criterion
+ * tols[j], one untimed and one timed solve of op_d's system with right-hand
+ * side b (x: a field of op_d, overwritten); times[i * n_tol + j] = seconds of
+ * the timed solve, -1 if it did not reach stop_cond.  *out = the Richardson
+ * solver of the fastest candidate, *best = its index. */
+int dwf_solver_tune_richardson(dwf_fopr op_d, int n_inner, const dwf_fopr* inner, int n_tol,
+                               const double* tols, int max_iter, double stop_cond,
+                               int max_iter_inner, const void* b, void* x, double* times,
+                               int* best, dwf_solver* out);
 /* solve(x, b, nconv, diff): diff = |A x - b| / |b| recomputed in the solver's
  * outer precision.  Returns DWF_ERR_NOT_CONVERGED (nconv = -1) when the
  * iteration limit is hit; x then holds the last iterate. */

@@ -19,6 +19,7 @@
 #include <mutex>
 #include <utility>
 #include <string>
+#include <chrono>
 #include <vector>
 
 #include "dwf_hip.h"
@@ -1596,6 +1597,60 @@ int dwf_solver_create_richardson(dwf_fopr op_d, dwf_fopr op_f, int max_iter, dou
   s->inner->guess = DWF_GUESS_ZERO;
   s->inner->fast = true;
   *out = s;
+  return DWF_OK;
+}
+
+// Schedule autotuner of the mixed-precision solve: for every inner operator
+// (storage type) and inner criterion, one untimed and one timed solve of the
+// caller's system; the solver of the fastest candidate that reached stop_cond
+// is returned, the others are destroyed.
+int dwf_solver_tune_richardson(dwf_fopr op_d, int n_inner, const dwf_fopr* inner, int n_tol,
+                               const double* tols, int max_iter, double stop_cond,
+                               int max_iter_inner, const void* b, void* x, double* times,
+                               int* best, dwf_solver* out) {
+  CTX();
+  if (!op_d || n_inner < 1 || !inner || n_tol < 1 || !tols || !b || !x || !times || !best ||
+      !out)
+    return fail(DWF_ERR_INVALID_ARG, "invalid argument");
+  *out = nullptr;
+  *best = -1;
+  double tbest = 0.0;
+  for (int i = 0; i < n_inner; ++i)
+    for (int j = 0; j < n_tol; ++j) {
+      const int c = i * n_tol + j;
+      times[c] = -1.0;
+      dwf_solver sv = nullptr;
+      RC(dwf_solver_create_richardson(op_d, inner[i], max_iter, stop_cond, max_iter_inner,
+                                      tols[j], &sv));
+      int nconv = -1;
+      double diff = 0.0;
+      int rc = dwf_solver_solve(sv, x, b, &nconv, &diff);   // fields allocated here
+      double dt = -1.0;
+      if (rc == DWF_OK && nconv >= 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = dwf_solver_solve(sv, x, b, &nconv, &diff);
+        HIPCHK(hipStreamSynchronize(g_ctx.stream));
+        dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
+      if (rc != DWF_OK && rc != DWF_ERR_NOT_CONVERGED) {
+        dwf_solver_destroy(sv);
+        if (*out) dwf_solver_destroy(*out);
+        *out = nullptr;
+        return rc;
+      }
+      if (rc == DWF_OK && nconv >= 0 && dt >= 0.0) {
+        times[c] = dt;
+        if (*best < 0 || dt < tbest) {
+          if (*out) dwf_solver_destroy(*out);
+          *out = sv;
+          *best = c;
+          tbest = dt;
+          continue;
+        }
+      }
+      dwf_solver_destroy(sv);
+    }
+  if (*best < 0) return fail(DWF_ERR_NOT_CONVERGED, "no candidate reached the stopping condition");
   return DWF_OK;
 }
 

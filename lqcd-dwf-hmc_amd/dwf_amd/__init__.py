@@ -292,6 +292,36 @@ class ASolver_Richardson(ASolver_CG):
         self.fopr_f = fopr_f
 
 
+class _SolverHandle(ASolver_CG):
+    """A solver created by the library (dwf_solver_tune_richardson)."""
+
+    def __init__(self, handle, fopr, fopr_f=None):
+        self._h = handle
+        self.fopr = fopr
+        self.fopr_f = fopr_f
+
+
+def tune_richardson(fopr_d, inner_foprs, inner_criteria, b, x, maximum_number_of_iteration=100,
+                    convergence_criterion_squared=1.0e-20, inner_iteration=100000):
+    """Schedule autotuner of the mixed-precision solve (no counterpart in the
+    reference): times one solve of fopr_d's system (right-hand side b, x
+    overwritten) per inner operator (storage type) and inner criterion, after an
+    untimed one, and returns (solver of the fastest candidate that converged,
+    times[i][j] in seconds or -1, (i, j) of the choice)."""
+    ops = list(inner_foprs)
+    tols = np.ascontiguousarray(inner_criteria, np.float64)
+    arr = (C.c_void_p * len(ops))(*[o._h for o in ops])
+    times = np.zeros(len(ops) * len(tols))
+    best = C.c_int(-1)
+    h = C.c_void_p()
+    _lib.check("dwf_solver_tune_richardson", _lib.load().dwf_solver_tune_richardson(
+        fopr_d._h, len(ops), arr, len(tols), _ptr(tols), int(maximum_number_of_iteration),
+        float(convergence_criterion_squared), int(inner_iteration), b.ptr, x.ptr, _ptr(times),
+        C.byref(best), C.byref(h)))
+    i, j = divmod(best.value, len(tols))
+    return _SolverHandle(h.value, fopr_d, ops[i]), times.reshape(len(ops), len(tols)), (i, j)
+
+
 class APrecond_LearnedPolynomial:
     """Learned polynomial preconditioner K = sum_k W_k H^k, H = 1 - D (no
     counterpart in the reference).  Applied in fopr's precision; trained by

@@ -99,6 +99,9 @@ _SIGS = {
     "dwf_solver_create_cg": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.POINTER(_vp)]),
     "dwf_solver_create_richardson": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
                                                C.c_double, C.POINTER(_vp)]),
+    "dwf_solver_tune_richardson": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int, _dp, C.c_int,
+                                             C.c_double, C.c_int, _vp, _vp, _dp,
+                                             C.POINTER(C.c_int), C.POINTER(_vp)]),
     "dwf_solver_solve": (C.c_int, [_vp, _vp, _vp, _ip, _dp]),
     "dwf_solver_inner_iterations": (C.c_int, [_vp]),
     "dwf_solver_destroy": (C.c_int, [_vp]),

@@ -231,6 +231,33 @@ def test_richardson_low_precision_inner_matrix_core(prm, inner):
     assert rel(x, gpu.reverse(fy)) < 1e-7
 
 
+def test_schedule_autotuner_picks_a_converging_fastest_candidate():
+    """dwf_solver_tune_richardson: every candidate (inner storage x inner
+    criterion) is timed; the returned solver is the fastest one that converged
+    and solves the system to the FP64 stopping condition."""
+    prm = LS16[1]
+    orc, gpu, U = pair(prm)
+    inner = []
+    for prec in ("half", "float"):
+        g = dwf_amd.AFopr_Domainwall_5din_eo(prm[0], prm[1], prm[2], prm[3], prm[4], prm[5],
+                                             prm[6], prec)
+        g.set_config(U)
+        inner.append(g)
+    gpu.set_mode("DdagD")
+    b = lattice.random_spinor5(prm[0], prm[1], 23)
+    fb, fx = gpu.new_field(), gpu.new_field()
+    gpu.convert(fb, b)
+    solver, times, (i, j) = dwf_amd.tune_richardson(gpu, inner, [1e-4, 1e-3], fb, fx,
+                                                    100, 1e-24, 10000)
+    print("  tuner times", times, "choice", (i, j), flush=True)
+    assert times[i, j] > 0 and times[i, j] == times[times > 0].min()
+    fy = gpu.new_field()
+    n, d = solver.solve(fy, fb)
+    assert n >= 1 and d < 1e-12
+    x = gpu.reverse(fy)
+    assert np.linalg.norm(orc.mult("DdagD", x) - b) / np.linalg.norm(b) < 1e-11
+
+
 def test_multishift_matches_oracle():
     prm = ((4, 4, 4, 4), 6, 0.05, 1.6, 1.0, 0.0, (1, 1, 1, -1))
     orc, gpu, U = pair(prm)
