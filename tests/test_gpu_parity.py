@@ -209,7 +209,8 @@ def test_richardson_mixed_precision_matches_oracle():
 
 
 @pytest.mark.parametrize("inner", ["half", "float", "bfloat16", "double"])
-@pytest.mark.parametrize("prm", LS16 + LS12, ids=["ls16-shamir", "ls16-mobius", "ls12-mobius"])
+@pytest.mark.parametrize("prm", LS16 + LS12 + [((4, 4, 4, 8), 12, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1))],
+                         ids=["ls16-shamir", "ls16-mobius", "ls12-mobius", "ls12-shamir"])
 def test_richardson_low_precision_inner_matrix_core(prm, inner):
     """Mixed solve with the sync-free inner CG at Ls = 16 and 12 (matrix-core
     epilogue, fused p update + Mp prologue): the FP64 residual reaches the
