@@ -117,6 +117,55 @@ def cpu_baseline(U_smeared, x, seconds):
     }
 
 
+def cpu_kernels_and_cfg1(dwf_amd, lattice, np, U_smeared, x, log):
+    """Beside the D_eo figure: the oracle's FP64 DdagD at 16^3x32, Ls=16, and the
+    reference's CPU-runnable case (8^4, Ls=8 Shamir, M0=1.6, mq=0.02, exp(0.5 iH)
+    links with 2 stout steps, FP64 CG to |r|/|b| < 1e-14) solved by the oracle on
+    the host and by libdwfhip on the GPU."""
+    from oracle import OracleDWF
+    orc = OracleDWF(DIMS, LS, MQ, M0, B, CC)
+    orc.set_config(U_smeared)
+    orc.mult("DdagD", x)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        orc.mult("DdagD", x)
+    ddd_ms = (time.perf_counter() - t0) / 3 * 1e3
+    dims1, ls1 = (8, 8, 8, 8), 8
+    U1 = lattice.random_gauge(dims1, 0.5, 2024)
+    sm1 = dwf_amd.Director_alt_Smear(dims1, 0.1, 2)
+    sm1.set_config(U1)
+    Us1 = sm1.get_config(2)
+    b1 = lattice.random_spinor5(dims1, ls1, 7)
+    o1 = OracleDWF(dims1, ls1, 0.02, 1.6, 1.0, 0.0)
+    o1.set_config(Us1)
+    t0 = time.perf_counter()
+    xo, no, do = o1.cg(b1, "DdagD", 10000, 1e-28)
+    t_cpu = time.perf_counter() - t0
+    g1 = dwf_amd.AFopr_Domainwall_5din_eo(dims1, ls1, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1), "double")
+    g1.set_config_smeared(sm1, None)
+    g1.set_mode("DdagD")
+    fb, fx = g1.new_field(), g1.new_field()
+    g1.convert(fb, b1)
+    cg = dwf_amd.ASolver_CG(g1, 10000, 1e-28)
+    cg.solve(fx, fb)
+    dwf_amd.synchronize()
+    t0 = time.perf_counter()
+    ng, dg = cg.solve(fx, fb)
+    dwf_amd.synchronize()
+    t_gpu = time.perf_counter() - t0
+    xg = g1.reverse(fx)
+    out = {"oracle_ddagd_ms_16x16x16x32_Ls16": round(ddd_ms, 1),
+           "cfg1_fp64_cg_8x8x8x8_Ls8": {
+               "cpu_oracle_s": round(t_cpu, 4), "cpu_iterations": no,
+               "gpu_s": round(t_gpu, 4), "gpu_iterations": ng,
+               "gpu_true_residual": dg, "rel_diff_solutions":
+                   float(np.linalg.norm(xg - xo) / np.linalg.norm(xo))}}
+    log(f"CPU oracle DdagD {ddd_ms:.1f} ms; config 1 CG: CPU {t_cpu:.3f} s ({no} it), "
+        f"GPU {t_gpu:.4f} s ({ng} it), solutions differ by "
+        f"{out['cfg1_fp64_cg_8x8x8x8_Ls8']['rel_diff_solutions']:.1e}")
+    return out
+
+
 def bench_batch(dwf_amd, lattice, np, f64, f32, sm, log):
     """Batched multishift (16 shifts log-spaced on [1e-4, 10], |r|^2/|b|^2 < 1e-20):
     4 Gaussian right-hand sides on 16^3x32, Ls=16 batched against one multishift
@@ -477,6 +526,7 @@ def main():
     if not a.no_cpu_baseline:
         Us = sm.get_config(NSMEAR)
         cpu = cpu_baseline(Us, b_host, a.cpu_seconds)
+        cpu.update(cpu_kernels_and_cfg1(dwf_amd, lattice, np, Us, b_host, log))
         log(f"CPU baseline: {cpu}")
 
     traffic = load_traffic("k_dwf_hop_f64_16x16x16x32_Ls16")
