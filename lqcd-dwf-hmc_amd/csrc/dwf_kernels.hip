@@ -513,18 +513,28 @@ template <typename T, int OP> constexpr int k_waves() {
   return (OP == K_SITE && sizeof(T) == 2) ? 3 : KCfg<T>::WAVES;
 }
 
-// Sites per block of a compiled-Ls kernel: THREADS / (Ls / VEC), at Ls = 12 a
-// multiple of 16 (whole waves of 12-thread sites whose 12 spinor rows per
-// chirality fill 16-column matrix-core tiles).  The launchers use the same rule.
+// Threads per site of a compiled-Ls kernel: Ls / VEC, except 16-bit storage at
+// Ls = 12, whose 6 threads per site are padded to 8 (the last two idle).  With 6,
+// two of every three 4-lane groups of a gather span two sites, whose 24-byte
+// rows then do not coalesce: the hop made 4.6x the L1 accesses of Ls = 16 and
+// ran at half its bytes per second.
+template <typename T> __host__ __device__ constexpr int tps_of(int Ls, int vec) {
+  return (sizeof(T) == 2 && Ls == 12 && vec == 2) ? 8 : Ls / vec;
+}
+// Sites per block of a compiled-Ls kernel: THREADS / tps, at Ls = 12 a multiple
+// of 16 (whole waves of sites whose 12 spinor rows per chirality fill 16-column
+// matrix-core tiles).  The launchers use the same rule.
 template <typename T> __host__ __device__ constexpr int spb_of(int Ls, int vec) {
-  return Ls == 12 ? (KCfg<T>::THREADS / (12 / vec)) / 16 * 16 : KCfg<T>::THREADS / (Ls / vec);
+  return Ls == 12 ? (KCfg<T>::THREADS / tps_of<T>(12, vec)) / 16 * 16
+                  : KCfg<T>::THREADS / tps_of<T>(Ls, vec);
 }
 // the chiral epilogue on the matrix cores (Ls x Ls in a 16 x 16 tile)
 constexpr bool mfma_ls(int Ls) { return Ls == 16 || Ls == 12; }
 
-// LSC > 0: Ls and the block shape (THREADS / (LSC / VEC) sites) are compile-time
-// constants, so every load and LDS access uses an immediate offset; LSC = 0:
-// runtime Ls.  Thread (lsite, q) owns s = q*VEC .. q*VEC + VEC - 1 of its site.
+// LSC > 0: Ls and the block shape (spb_of sites of tps_of threads) are
+// compile-time constants, so every load and LDS access uses an immediate offset;
+// LSC = 0: runtime Ls.  Thread (lsite, q) owns s = q*VEC .. q*VEC + VEC - 1 of
+// its site (none when that is beyond Ls: padding threads).
 //
 // Several right-hand sides on one lattice (nr = 2^nrs of them, each a whole
 // field, rstride reals after the previous one) run as one launch: the kernel
@@ -553,7 +563,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
   // site launch: upd += ua X;  hop launch: upd -= ua out, stored instead of out,
   // and the norm taken of the new upd
   const C ua = upd ? (C)(unum[0] / uden[0]) : C(0);
-  const int Q = Ls / VEC;   // threads per site
+  const int Q = LSC > 0 ? tps_of<T>(LSC, VEC) : Ls / VEC;   // threads per site
   static_assert(KCfg<T>::THREADS % 64 == 0 && KCfg<T>::THREADS <= 512, "block of 1-8 waves");
   const int spb = LSC > 0 ? spb_of<T>(LSC, VEC) : spb_rt;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -593,7 +603,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
   const int rmask = (1 << nrs) - 1;
   const int site0 = blk * spb;
   const int site = site0 + lsite;
-  const bool valid = (lsite < spb) && (site < VhV);
+  const bool valid = (lsite < spb) && (site < VhV) && (s0 < Ls);   // s0 >= Ls: padding thread
   const int rsite0 = site0 >> nrs;
   const int vend = (site0 + spb < VhV ? site0 + spb : VhV) - 1;
   const int nsite = (vend >> nrs) - rsite0 + 1;
@@ -768,7 +778,7 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
     using v4 = typename MM::v4;
     constexpr int LP = 17;
     constexpr int SPB = spb_of<T>(LSC, VEC);
-    constexpr int NW = SPB * (LSC / VEC) / 64;     // waves per block
+    constexpr int NW = SPB * tps_of<T>(LSC, VEC) / 64;   // waves per block
     constexpr int NT = SPB * 12 / 16;              // 16-column tiles per block
     constexpr int NTW = (NT + NW - 1) / NW;        // tiles per wave
     constexpr int LL = LSC * LSC;                  // one chirality's matrix
@@ -1323,7 +1333,7 @@ static hipError_t launch_dwf_t(int op, void* out, const void* in, const void* li
   const int VhV = Vh << e.nrs;   // virtual sites (site, rhs)
   if (lsc && lds_for(spb) <= lds_cap) {
     const size_t lds = lds_for(spb);
-    const dim3 block(spb * (Ls / vec)), grid((VhV + spb - 1) / spb);
+    const dim3 block(spb * tps_of<T>(Ls, vec)), grid((VhV + spb - 1) / spb);
     if (e.fin_partial && block.x % 64) return hipErrorInvalidValue;   // block_sum: whole waves
     *nblocks = grid.x;
     switch (Ls) {
@@ -1443,7 +1453,7 @@ static hipError_t prepare_group_t(int op, int Ls, int n, const GroupMember* m, G
   gl->addx = addx;
   gl->out2 = out2;
   gl->grid = begin;
-  gl->block = spb * (Ls / KCfg<T>::VEC);
+  gl->block = spb * tps_of<T>(Ls, KCfg<T>::VEC);
   hipError_t err = hipMalloc(&gl->table, sizeof(DwfArgs<T>) * n);
   if (err != hipSuccess) return err;
   return hipMemcpy(gl->table, tab.data(), sizeof(DwfArgs<T>) * n, hipMemcpyHostToDevice);
