@@ -513,13 +513,19 @@ template <typename T, int OP> constexpr int k_waves() {
   return (OP == K_SITE && sizeof(T) == 2) ? 3 : KCfg<T>::WAVES;
 }
 
-// Threads per site of a compiled-Ls kernel: Ls / VEC, except 16-bit storage at
-// Ls = 12, whose 6 threads per site are padded to 8 (the last two idle).  With 6,
-// two of every three 4-lane groups of a gather span two sites, whose 24-byte
-// rows then do not coalesce: the hop made 4.6x the L1 accesses of Ls = 16 and
-// ran at half its bytes per second.
+// Threads per site of a compiled-Ls kernel: Ls / VEC, except at Ls = 12, where
+// the last threads of a site idle:
+// * 16-bit storage: 6 threads per site padded to 8.  With 6, two of every three
+//   4-lane groups of a gather span two sites, whose 24-byte rows then do not
+//   coalesce: the hop made 4.6x the L1 accesses of Ls = 16 and ran at half its
+//   bytes per second.
+// * FP32: 12 padded to 16, so that a block of 32 sites is 8 waves: blocks of 6
+//   waves fit only twice in a CU under the 4-waves-per-SIMD register budget
+//   (D_eo at 16^3x32: 70.9 -> 57.0 us).
 template <typename T> __host__ __device__ constexpr int tps_of(int Ls, int vec) {
-  return (sizeof(T) == 2 && Ls == 12 && vec == 2) ? 8 : Ls / vec;
+  return (Ls == 12 && sizeof(T) == 2 && vec == 2)   ? 8
+         : (Ls == 12 && sizeof(T) == 4 && vec == 1) ? 16
+                                                    : Ls / vec;
 }
 // Sites per block of a compiled-Ls kernel: THREADS / tps, at Ls = 12 a multiple
 // of 16 (whole waves of sites whose 12 spinor rows per chirality fill 16-column
