@@ -2035,18 +2035,34 @@ int dwf_shiftsolver_cg_solve(dwf_fopr op, int max_iter, double stop_cond, int ns
     double alpha_p = 0.0, beta_p = 1.0;
     int Nconv2 = -1;
     bool converged = false;
+    // DdagD: <p0, (DdagD + sigma0) p0> = |D p0|^2 + sigma0 |p0|^2, the first norm
+    // fused into the last hop of D and the second into the update of p0, and
+    // sigma0 p0 added in the residual update: no separate pass over s and p0.
+    // Other modes: s += sigma0 p0 and <p0, s> in one pass, as the reference.
+    const bool fused = op->mode == "DdagD";
+    double pp0 = bn;   // |p0|^2 (p0 = b at the start)
     for (int iter = 0; iter < max_iter; ++iter) {
-      RC(mult_impl(op, s, p[0], op->mode, false));
       double pa_p;
-      HIPCHK(launch_axpy_dot(P64, s, sigma0, p[0], n, g_ctx.red, 0, st));
-      RC(fetch(1, &pa_p));
+      if (fused) {
+        RC(op_DdagD_nrm(op, s, p[0], g_ctx.red.result));
+        double v[2];
+        RC(fetch(iter == 0 ? 1 : 2, v));
+        if (iter > 0) pp0 = v[1];
+        pa_p = v[0] + sigma0 * pp0;
+      } else {
+        RC(mult_impl(op, s, p[0], op->mode, false));
+        HIPCHK(launch_axpy_dot(P64, s, sigma0, p[0], n, g_ctx.red, 0, st));
+        RC(fetch(1, &pa_p));
+      }
       const double rr_p = rr;
       const double beta = -rr_p / pa_p;
       ++m;
-      HIPCHK(launch_cg_xr_to(P64, xq[0], R[m], R[m - 1], p[0], s, -beta, n, g_ctx.red, 0, st));
+      HIPCHK(launch_cg_xr_to(P64, xq[0], R[m], R[m - 1], p[0], s, -beta, fused ? sigma0 : 0.0, n,
+                             g_ctx.red, 0, st));
       RC(fetch(1, &rr));
       const double alpha = rr / rr_p;
-      HIPCHK(launch_aypx(P64, alpha, p[0], R[m], n, st));
+      if (fused) HIPCHK(launch_aypx_norm(P64, alpha, p[0], R[m], n, g_ctx.red, 1, st));
+      else HIPCHK(launch_aypx(P64, alpha, p[0], R[m], n, st));
       pp[0] = rr;
       const double alpha_h = 1.0 + alpha_p * beta / beta_p;
       for (int ish = 1; ish < Nshift2; ++ish) {

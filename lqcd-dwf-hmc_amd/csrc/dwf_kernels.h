@@ -211,10 +211,13 @@ hipError_t launch_dot(int prec, const void* x, const void* y, size_t n, const Re
 hipError_t launch_dotc_ls(int prec, const void* x, const void* y, size_t n, int Ls,
                           const RedScratch& r, int slot, hipStream_t st);
 // CG step:  x += a p;  r -= a s;  result[slot] = |r|^2
-// x += a p, r_out = r_in - a s, |r_out|^2 -> result[slot] (r_out != r_in)
+// x += a p, r_out = r_in - a (s + sig p), |r_out|^2 -> result[slot] (r_out != r_in)
 hipError_t launch_cg_xr_to(int prec, void* x, void* rout, const void* rin, const void* p,
-                           const void* s, double a, size_t n, const RedScratch& red, int slot,
-                           hipStream_t st);
+                           const void* s, double a, double sig, size_t n, const RedScratch& red,
+                           int slot, hipStream_t st);
+// y = a y + x, |y|^2 -> result[slot]
+hipError_t launch_aypx_norm(int prec, double a, void* y, const void* x, size_t n,
+                            const RedScratch& red, int slot, hipStream_t st);
 hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s, double a,
                         size_t n, const RedScratch& red, int slot, hipStream_t st);
 // y += a x; result[slot] = Re<y, x>   (used as s += sigma0 p; pap = <s, p>)

@@ -2106,12 +2106,13 @@ k_cg_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* 
   }
   block_finish<1>(v, partial, counter, result);
 }
-// x += a p;  r_out = r_in - a s (r_out another buffer);  |r_out|^2
+// x += a p;  r_out = r_in - a (s + sig p) (r_out another buffer);  |r_out|^2
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_cg_xr_to(T* __restrict__ x, T* __restrict__ rout, const T* __restrict__ rin,
-           const T* __restrict__ p, const T* __restrict__ s, typename St<T>::C a, size_t n,
-           double* partial, unsigned* counter, double* result) {
+           const T* __restrict__ p, const T* __restrict__ s, typename St<T>::C a,
+           typename St<T>::C sig, size_t n, double* partial, unsigned* counter,
+           double* result) {
   using C = typename St<T>::C;
   double v[1] = {0};
   PACK_LOOP(T, n) {
@@ -2123,17 +2124,43 @@ k_cg_xr_to(T* __restrict__ x, T* __restrict__ rout, const T* __restrict__ rin,
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       xv[j] += a * pv[j];
-      rv[j] = rounded<T>(rv[j] - a * sv[j]);
+      rv[j] = rounded<T>(rv[j] - a * (sv[j] + sig * pv[j]));
       v[0] += (double)rv[j] * (double)rv[j];
     }
     Vec<T>::st(x + ip * NV, xv);
     Vec<T>::st(rout + ip * NV, rv);
   }
   TAIL_LOOP(n) {
-    St<T>::st(x + i, St<T>::ld(x + i) + a * St<T>::ld(p + i));
-    C rn = rounded<T>(St<T>::ld(rin + i) - a * St<T>::ld(s + i));
+    const C pv = St<T>::ld(p + i);
+    St<T>::st(x + i, St<T>::ld(x + i) + a * pv);
+    C rn = rounded<T>(St<T>::ld(rin + i) - a * (St<T>::ld(s + i) + sig * pv));
     St<T>::st(rout + i, rn);
     v[0] += (double)rn * (double)rn;
+  }
+  block_finish<1>(v, partial, counter, result);
+}
+// y = a y + x;  |y|^2 of the stored y
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_aypx_norm(typename St<T>::C a, T* __restrict__ y, const T* __restrict__ x, size_t n,
+            double* partial, unsigned* counter, double* result) {
+  using C = typename St<T>::C;
+  double v[1] = {0};
+  PACK_LOOP(T, n) {
+    C xv[NV], yv[NV];
+    Vec<T>::ld(x + ip * NV, xv);
+    Vec<T>::ld(y + ip * NV, yv);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      yv[j] = rounded<T>(a * yv[j] + xv[j]);
+      v[0] += (double)yv[j] * (double)yv[j];
+    }
+    Vec<T>::st(y + ip * NV, yv);
+  }
+  TAIL_LOOP(n) {
+    const C yv = rounded<T>(a * St<T>::ld(y + i) + St<T>::ld(x + i));
+    St<T>::st(y + i, yv);
+    v[0] += (double)yv * (double)yv;
   }
   block_finish<1>(v, partial, counter, result);
 }
@@ -2211,10 +2238,18 @@ static void cg_xr_t(void* x, void* rr, const void* p, const void* s, double a, s
 }
 template <typename T>
 static void cg_xr_to_t(void* x, void* rout, const void* rin, const void* p, const void* s,
-                       double a, size_t n, const RedScratch& r, int slot, hipStream_t st) {
+                       double a, double sig, size_t n, const RedScratch& r, int slot,
+                       hipStream_t st) {
   hipLaunchKernelGGL(k_cg_xr_to<T>, red_grid(n), dim3(256), 0, st, (T*)x, (T*)rout,
-                     (const T*)rin, (const T*)p, (const T*)s, (typename St<T>::C)a, n, r.partial,
-                     r.counter, r.result + slot);
+                     (const T*)rin, (const T*)p, (const T*)s, (typename St<T>::C)a,
+                     (typename St<T>::C)sig, n, r.partial, r.counter, r.result + slot);
+  red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
+}
+template <typename T>
+static void aypx_norm_t(double a, void* y, const void* x, size_t n, const RedScratch& r, int slot,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_aypx_norm<T>, red_grid(n), dim3(256), 0, st, (typename St<T>::C)a, (T*)y,
+                     (const T*)x, n, r.partial, r.counter, r.result + slot);
   red_finish<1>(r.partial, red_grid(n).x, r.result + slot, st);
 }
 template <typename T>
@@ -2262,9 +2297,14 @@ hipError_t launch_cg_xr(int prec, void* x, void* r, const void* p, const void* s
 }
 
 hipError_t launch_cg_xr_to(int prec, void* x, void* rout, const void* rin, const void* p,
-                           const void* s, double a, size_t n, const RedScratch& red, int slot,
-                           hipStream_t st) {
-  DWF_DISPATCH(prec, cg_xr_to_t, (x, rout, rin, p, s, a, n, red, slot, st));
+                           const void* s, double a, double sig, size_t n, const RedScratch& red,
+                           int slot, hipStream_t st) {
+  DWF_DISPATCH(prec, cg_xr_to_t, (x, rout, rin, p, s, a, sig, n, red, slot, st));
+  return hipGetLastError();
+}
+hipError_t launch_aypx_norm(int prec, double a, void* y, const void* x, size_t n,
+                            const RedScratch& red, int slot, hipStream_t st) {
+  DWF_DISPATCH(prec, aypx_norm_t, (a, y, x, n, red, slot, st));
   return hipGetLastError();
 }
 
