@@ -789,6 +789,21 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
     constexpr int NTW = (NT + NW - 1) / NW;        // tiles per wave
     constexpr int LL = LSC * LSC;                  // one chirality's matrix
     C* mb = shA;
+    // 16-bit storage: the staged columns are kept in the storage type, which is
+    // what the 16-bit matrix cores take (the same rounding as converting each
+    // operand at the tile), LPH halves apart so that a lane's 4 rows are one
+    // 8-byte read; rows beyond Ls are never staged and read as zero
+    constexpr bool S16 = sizeof(T) == 2 && DWF_EPI_MFMA16;
+    constexpr int LPH = 20;
+    unsigned short* mh = reinterpret_cast<unsigned short*>(shA);
+    auto h16 = [](C f) -> unsigned {
+      if constexpr (std::is_same<T, __half>::value) {
+        return __builtin_bit_cast(unsigned short, (_Float16)f);
+      } else {
+        const __hip_bfloat16 h = __float2bfloat16(f);
+        return *reinterpret_cast<const unsigned short*>(&h);
+      }
+    };
     const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int ntiles = (spb * 12) / 16;
@@ -803,14 +818,16 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
         // (lane l holds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]);
         // the output layout is that of the f32 16x16x4 instruction
         typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
-        h4_t av, bv;
+        h4_t av;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int sin = 4 * lk + i;
           av[i] = (_Float16)(sin < LSC && arow < LSC ? M[sin * LSC + arow] : C(0));
-          bv[i] = (_Float16)(sin < LSC ? mb[col * LP + sin] : C(0));
         }
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, init, 0, 0, 0);
+        const u32x2_t bw = 4 * lk < LSC ? *reinterpret_cast<const u32x2_t*>(mh + col * LPH + 4 * lk)
+                                        : u32x2_t{0u, 0u};
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(av, __builtin_bit_cast(h4_t, bw), init, 0, 0,
+                                                     0);
       } else if constexpr (std::is_same<T, __hip_bfloat16>::value && DWF_EPI_MFMA16) {
         // BF16 storage: the same with v_mfma_f32_16x16x16_bf16 (round to nearest even)
         typedef short s4_t __attribute__((ext_vector_type(4)));
@@ -818,14 +835,16 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
           const __hip_bfloat16 h = __float2bfloat16(f);
           return *reinterpret_cast<const short*>(&h);
         };
-        s4_t av, bv;
+        s4_t av;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int sin = 4 * lk + i;
           av[i] = bf(sin < LSC && arow < LSC ? M[sin * LSC + arow] : C(0));
-          bv[i] = bf(sin < LSC ? mb[col * LP + sin] : C(0));
         }
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, init, 0, 0, 0);
+        const u32x2_t bw = 4 * lk < LSC ? *reinterpret_cast<const u32x2_t*>(mh + col * LPH + 4 * lk)
+                                        : u32x2_t{0u, 0u};
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, __builtin_bit_cast(s4_t, bw), init, 0,
+                                                         0, 0);
       } else {
         v4 a = init;
 #pragma unroll
@@ -848,10 +867,17 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
     auto stage_acc = [&](C sign) {
       if (valid) {
 #pragma unroll
-        for (int k = 0; k < 12; ++k)
+        for (int k = 0; k < 12; ++k) {
+          if constexpr (S16) {   // VEC = 2: s0 and s0 + 1 in one word
+            *reinterpret_cast<unsigned*>(mh + (lsite * 12 + k) * LPH + s0) =
+                h16(acc[0][k] + sign * acc[0][k + 12]) |
+                (h16(acc[1][k] + sign * acc[1][k + 12]) << 16);
+          } else {
 #pragma unroll
-          for (int v = 0; v < VEC; ++v)
-            mb[(lsite * 12 + k) * LP + s0 + v] = acc[v][k] + sign * acc[v][k + 12];
+            for (int v = 0; v < VEC; ++v)
+              mb[(lsite * 12 + k) * LP + s0 + v] = acc[v][k] + sign * acc[v][k + 12];
+          }
+        }
       }
     };
     stage_acc(C(1));
@@ -868,9 +894,15 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
           for (int k = 0; k < 12; ++k) {
             const typename VL::R ra = VL::ld(X + base + k * Ls);
             const typename VL::R rb = VL::ld(X + base + (k + 12) * Ls);
+            if constexpr (S16) {
+              *reinterpret_cast<unsigned*>(mh + (lsite * 12 + k) * LPH + s0) =
+                  h16(VL::get(ra, 0) + sign * VL::get(rb, 0)) |
+                  (h16(VL::get(ra, 1) + sign * VL::get(rb, 1)) << 16);
+            } else {
 #pragma unroll
-            for (int v = 0; v < VEC; ++v)
-              mb[(lsite * 12 + k) * LP + s0 + v] = VL::get(ra, v) + sign * VL::get(rb, v);
+              for (int v = 0; v < VEC; ++v)
+                mb[(lsite * 12 + k) * LP + s0 + v] = VL::get(ra, v) + sign * VL::get(rb, v);
+            }
           }
         }
       };
@@ -957,13 +989,21 @@ dwf_body(T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ li
           // The FP32/FP64 chain reads rows written by other lanes of the wave,
           // and the second staging overwrites rows other lanes have just read:
           // each hand-over is a wave-scope release/acquire around a wave barrier.
+          auto stage_col = [&](C sign) {   // this lane's 4 rows of column col
+            if constexpr (S16) {
+              *reinterpret_cast<u32x2_t*>(mh + col * LPH + 4 * lk) =
+                  u32x2_t{h16(o1[0] + sign * o2[0]) | (h16(o1[1] + sign * o2[1]) << 16),
+                          h16(o1[2] + sign * o2[2]) | (h16(o1[3] + sign * o2[3]) << 16)};
+            } else {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] + o2[r];
+              for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] + sign * o2[r];
+            }
+          };
+          stage_col(C(1));
           wave_lds_sync();
           const v4 fu = mma_tile(sh_F, col, v4{0, 0, 0, 0});
           wave_lds_sync();
-#pragma unroll
-          for (int r = 0; r < 4; ++r) mb[col * LP + 4 * lk + r] = o1[r] - o2[r];
+          stage_col(C(-1));
           wave_lds_sync();
           const v4 fl = mma_tile(sh_F + LL, col, v4{0, 0, 0, 0});
           if (cvalid) {
