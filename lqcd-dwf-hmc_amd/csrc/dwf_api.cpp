@@ -2345,9 +2345,10 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
                             m->stages[i].data(), &m->glaunch[i]) != hipSuccess)
         return bad(DWF_ERR_UNSUPPORTED, "grouped launch could not be prepared");
   }
-  // device tables: s p0 x0 [nsys], ring [nsys][kFoldMax + 1], x p [nsys][32], n [nsys]
+  // device tables: s p0 x0 [nsys], ring [nsys][kFoldMax + 1], x p [nsys][32], u [nsys],
+  // n [nsys]
   constexpr int NR = kFoldMax + 1;
-  const size_t ntab = 3 * nsys + NR * nsys + 2 * nsys * kMsShifts;
+  const size_t ntab = 3 * nsys + NR * nsys + 2 * nsys * kMsShifts + nsys;
   const size_t ptab = sizeof(void*) * ntab;
   if (hipMalloc(&m->dtab, ptab + sizeof(size_t) * nsys) != hipSuccess)
     return bad(DWF_ERR_HIP, "device allocation failed");
@@ -2363,6 +2364,7 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
   m->L.ring = t + 3 * nsys;
   m->L.x = t + 3 * nsys + NR * nsys;
   m->L.p = t + 3 * nsys + NR * nsys + nsys * kMsShifts;
+  m->L.u = t + 3 * nsys + NR * nsys + 2 * nsys * kMsShifts;
   m->L.n = (const size_t*)((char*)m->dtab + ptab);
   if (hipMalloc(&m->L.state, sizeof(MsState) * nsys) != hipSuccess ||
       hipMalloc(&m->L.partial, sizeof(double) * nsys * kMsShifts * m->L.nblk) != hipSuccess ||
@@ -2385,6 +2387,7 @@ int dwf_msbatch_create(int nsys, const dwf_fopr* ops, int nshift, int flags, dwf
     const size_t off = (size_t)m->slot[k].second * M.op->nelem * prec_bytes(m->prec);
     h[k] = (char*)M.buf[MB_S] + off;
     h[nsys + k] = (char*)M.buf[MB_P0] + off;
+    h[3 * nsys + NR * nsys + 2 * nsys * kMsShifts + k] = (char*)M.buf[MB_U] + off;   // D p0
     for (int q = 0; q < NR; ++q) h[3 * nsys + k * NR + q] = m->ring[k][q];
     for (int j = 1; j < nshift; ++j)
       h[3 * nsys + NR * nsys + nsys * kMsShifts + k * kMsShifts + j] = m->p[k][j];
@@ -2590,6 +2593,12 @@ int dwf_msbatch_solve(dwf_msbatch m, const double* sigma, void** x, const void* 
   HIPCHK(hipMemcpyAsync(m->L.x, hx.data() + nsys, sizeof(void*) * nsys * kMsShifts,
                         hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(m->L.state, hs.data(), sizeof(MsState) * nsys, hipMemcpyHostToDevice, st));
+  // |p0|^2 of the first iteration's pa: p0 = b (slot 0 of pp_res; the update of
+  // p0 writes it afterwards)
+  std::vector<double> hpp((size_t)nsys * kMsShifts, 0.0);
+  for (int k = 0; k < nsys; ++k) hpp[(size_t)k * kMsShifts] = hs[k].rr;
+  HIPCHK(hipMemcpyAsync(m->L.pp_res, hpp.data(), sizeof(double) * nsys * kMsShifts,
+                        hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   int cnt0 = 0;
   HIPCHK(hipMemcpy(&cnt0, m->dcounter, sizeof(int), hipMemcpyDeviceToHost));

@@ -8,14 +8,15 @@
 // synchronisation and can be replayed from a captured graph.
 //
 // Per iteration, for every active system k:
-//   [operator]  s_k = A p0_k                       (multi-RHS / grouped hops)
-//   k_ms_pa     s_k += sigma0 p0_k; pa_k = <s_k, p0_k>
-//   k_ms_xr     x0_k += (rr_k / pa_k) p0_k; r_new = r - (rr_k / pa_k) s_k into
-//               the next slot of the system's residual ring; |r_new|^2
+//   [operator]  s_k = A p0_k = D^+ u_k, u_k = D p0_k   (multi-RHS / grouped hops)
+//   k_ms_pa     pa_k = <p0_k, (A + sigma0) p0_k> = |u_k|^2 + sigma0 |p0_k|^2
+//               (|p0_k|^2 from the p0 update of the previous iteration)
+//   k_ms_xr     x0_k += (rr_k / pa_k) p0_k; r_new = r - (rr_k / pa_k) (s_k + sigma0 p0_k)
+//               into the next slot of the system's residual ring; |r_new|^2
 //   k_ms_ctrl   truncation of the shifts by the previous |p_j|^2, beta, alpha,
 //               zeta_j and the shifts' block coefficients, convergence, and
 //               whether this iteration ends a block
-//   k_ms_shift  p0_k = r + alpha p0_k; at the end of a block, for the shifts
+//   k_ms_shift  p0_k = r + alpha p0_k, |p0_k|^2; at the end of a block, for the shifts
 //               j >= 1 (k_ms_fold's combination):
 //               x_kj += b p_kj + sum_l d_l R_l; p_kj = a p_kj + sum_l c_l R_l; |p_kj|^2
 // so the shifts' fields are touched once per block of kFoldMax iterations
@@ -68,28 +69,20 @@ __device__ __forceinline__ double ms_block_sum(double v, double* sh) {
   for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 2; i < (n);        \
        i += (size_t)gridDim.x * blockDim.x * 2)
 
-// s += sigma0 p0;  <s, p0>  (fields of an even number of reals)
+// |u|^2, u = D p0 (fields of an even number of reals)
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_ms_pa(T* const* __restrict__ s, T* const* __restrict__ p0, const size_t* __restrict__ n,
+k_ms_pa(const T* const* __restrict__ u, const size_t* __restrict__ n,
         const MsState* __restrict__ st, double* __restrict__ partial) {
   using C = typename St<T>::C;
   const int k = blockIdx.y;
   if (!st[k].active) return;
-  const C sg = (C)st[k].sigma0;
-  T* sk = s[k];
-  const T* pk = p0[k];
+  const T* uk = u[k];
   double v = 0;
   MS_LOOP(n[k]) {
-    C sv[2], pv[2];
-    Vec2<T>::ld(sk + i, sv);
-    Vec2<T>::ld(pk + i, pv);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      sv[j] = (sv[j] + sg * pv[j]);
-      v += (double)sv[j] * (double)pv[j];
-    }
-    Vec2<T>::st(sk + i, sv);
+    C uv[2];
+    Vec2<T>::ld(uk + i, uv);
+    v += (double)uv[0] * (double)uv[0] + (double)uv[1] * (double)uv[1];
   }
   __shared__ double sh[4];
   const double t = ms_block_sum(v, sh);
@@ -107,6 +100,7 @@ k_ms_xr(T* const* __restrict__ x0, T* const* __restrict__ ring, T* const* __rest
   const MsState& S = st[k];
   if (!S.active) return;
   const C a = (C)(S.rr / S.pa);
+  const C sg = (C)S.sigma0;
   constexpr int NR = kFoldMax + 1;
   T* xk = x0[k];
   const T* rin = ring[(size_t)k * NR + (S.q0 + S.m) % NR];
@@ -123,7 +117,7 @@ k_ms_xr(T* const* __restrict__ x0, T* const* __restrict__ ring, T* const* __rest
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       xv[j] += a * pv[j];
-      rv[j] = (rv[j] - a * sv[j]);
+      rv[j] = (rv[j] - a * (sv[j] + sg * pv[j]));
       v += (double)rv[j] * (double)rv[j];
     }
     Vec2<T>::st(xk + i, xv);
@@ -136,8 +130,8 @@ k_ms_xr(T* const* __restrict__ x0, T* const* __restrict__ ring, T* const* __rest
 
 // p0 = r + alpha p0 (r the newest residual); at the end of a block also the
 // fold of shifts j = 1 .. fnfold - 1 with the block's residuals R_1 .. R_fm
-// and coefficients fc[cbuf ^ 1][j]; |p_j|^2 block partials at
-// partial[((k * kMsShifts) + j) * gridDim.x + block].
+// and coefficients fc[cbuf ^ 1][j]; |p_j|^2 block partials (j = 0: the new p0)
+// at partial[((k * kMsShifts) + j) * gridDim.x + block].
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_ms_shift(T* const* __restrict__ x, T* const* __restrict__ p, T* const* __restrict__ p0,
@@ -152,15 +146,22 @@ k_ms_shift(T* const* __restrict__ x, T* const* __restrict__ p, T* const* __restr
   T* p0k = p0[k];
   T* const* rk = ring + (size_t)k * NR;
   const T* rc = rk[S.rcur];
+  __shared__ double sh[4];
   if (!S.fold) {
+    double v0 = 0;
     MS_LOOP(n[k]) {
       C rv[2], qv[2];
       Vec2<T>::ld(rc + i, rv);
       Vec2<T>::ld(p0k + i, qv);
 #pragma unroll
-      for (int e = 0; e < 2; ++e) qv[e] = rv[e] + alpha * qv[e];
+      for (int e = 0; e < 2; ++e) {
+        qv[e] = rv[e] + alpha * qv[e];
+        v0 += (double)qv[e] * (double)qv[e];
+      }
       Vec2<T>::st(p0k + i, qv);
     }
+    const double t = ms_block_sum(v0, sh);
+    if (threadIdx.x == 0) partial[(size_t)k * kMsShifts * gridDim.x + blockIdx.x] = t;
     return;
   }
   const int nsh = S.fnfold, m = S.fm, q0 = S.fq0;
@@ -182,7 +183,10 @@ k_ms_shift(T* const* __restrict__ x, T* const* __restrict__ p, T* const* __restr
       C rn[2];   // the newest residual (R_fm, read again rather than indexed)
       Vec2<T>::ld(rc + i, rn);
 #pragma unroll
-      for (int e = 0; e < 2; ++e) qv[e] = rn[e] + alpha * qv[e];
+      for (int e = 0; e < 2; ++e) {
+        qv[e] = rn[e] + alpha * qv[e];
+        v[0] += (double)qv[e] * (double)qv[e];
+      }
     }
     Vec2<T>::st(p0k + i, qv);
 #pragma unroll
@@ -211,10 +215,9 @@ k_ms_shift(T* const* __restrict__ x, T* const* __restrict__ p, T* const* __restr
       }
     }
   }
-  __shared__ double sh[4];
 #pragma unroll
-  for (int j = 1; j < kMsShifts; ++j) {
-    if (j < nsh) {
+  for (int j = 0; j < kMsShifts; ++j) {
+    if (j == 0 || j < nsh) {
       const double t = ms_block_sum(v[j], sh);
       if (threadIdx.x == 0)
         partial[((size_t)k * kMsShifts + j) * gridDim.x + blockIdx.x] = t;
@@ -234,6 +237,19 @@ k_ms_finish(const double* __restrict__ partial, int nblk, int nout, double* out,
   __shared__ double sh[4];
   const double t = ms_block_sum(w, sh);
   if (threadIdx.x == 0) out[(size_t)k * ostride + o] = t;
+}
+// pa_k = (sum of the nblk partials of system k) + sigma0 |p0_k|^2, |p0_k|^2 =
+// pp_res[k][0] (the previous p0 update, or |b_k|^2 at the start); block k
+__global__ void __launch_bounds__(256)
+k_ms_finish_pa(const double* __restrict__ partial, int nblk, const double* __restrict__ pp_res,
+               MsState* __restrict__ st) {
+  const int k = blockIdx.x;
+  const double* p = partial + (size_t)k * nblk;
+  double w = 0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) w += p[i];
+  __shared__ double sh[4];
+  const double t = ms_block_sum(w, sh);
+  if (threadIdx.x == 0 && st[k].active) st[k].pa = t + st[k].sigma0 * pp_res[(size_t)k * kMsShifts];
 }
 
 // The scalar part of solve_step for every system, one wave per system with
@@ -350,10 +366,9 @@ static hipError_t ms_iter_blas_t(const MsLaunch& L, int stage, hipStream_t st) {
   const dim3 g(L.nblk, L.nsys), b(256);
   switch (stage) {
     case 0:
-      hipLaunchKernelGGL(k_ms_pa<T>, g, b, 0, st, (T* const*)L.s, (T* const*)L.p0, L.n, L.state,
-                         L.partial);
-      hipLaunchKernelGGL(k_ms_finish, dim3(1, L.nsys), b, 0, st, L.partial, L.nblk, 1,
-                         &L.state->pa, (int)(sizeof(MsState) / sizeof(double)));
+      hipLaunchKernelGGL(k_ms_pa<T>, g, b, 0, st, (const T* const*)L.u, L.n, L.state, L.partial);
+      hipLaunchKernelGGL(k_ms_finish_pa, dim3(L.nsys), b, 0, st, L.partial, L.nblk, L.pp_res,
+                         L.state);
       break;
     case 1:
       hipLaunchKernelGGL(k_ms_xr<T>, g, b, 0, st, (T* const*)L.x0, (T* const*)L.ring,

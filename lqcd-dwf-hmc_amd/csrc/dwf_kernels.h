@@ -106,9 +106,10 @@ struct MsState {
   double fc[2][kMsShifts][kFoldCoef];
 };
 // Device pointer tables of one batched solve (all in device memory):
-// s, p0, x0: [nsys]; ring: [nsys][kFoldMax + 1]; x, p: [nsys][kMsShifts] (slot 0
-// unused); n: reals per field of each system; partial: nsys * kMsShifts * nblk
-// doubles; pp_res: nsys * kMsShifts; status: coherent host memory (2 doubles).
+// s, p0, x0, u (= D p0, the operator's intermediate): [nsys]; ring: [nsys][kFoldMax + 1];
+// x, p: [nsys][kMsShifts] (slot 0 unused); n: reals per field of each system;
+// partial: nsys * kMsShifts * nblk doubles; pp_res: nsys * kMsShifts (slot 0: |p0|^2);
+// status: coherent host memory (2 doubles).
 struct MsLaunch {
   int nsys = 0, nblk = 0;
   void** s = nullptr;
@@ -117,6 +118,7 @@ struct MsLaunch {
   void** ring = nullptr;
   void** x = nullptr;
   void** p = nullptr;
+  void** u = nullptr;
   const size_t* n = nullptr;
   MsState* state = nullptr;
   double* partial = nullptr;
@@ -124,7 +126,8 @@ struct MsLaunch {
   double* status = nullptr;
   int* counter = nullptr;   // iterations done (device), copied to status[0]
 };
-// stage 0: s += sigma0 p0, pa;  1: x0/r update, rr_new;  2: p0 and the shifts
+// stage 0: pa = |u|^2 + sigma0 |p0|^2;  1: x0/r update, rr_new;  2: p0 (and |p0|^2)
+// and the shifts
 hipError_t launch_ms_blas(int prec, const MsLaunch& L, int stage, hipStream_t st);
 hipError_t launch_ms_ctrl(const MsLaunch& L, hipStream_t st);
 
