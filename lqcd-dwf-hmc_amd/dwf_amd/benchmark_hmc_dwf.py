@@ -39,6 +39,9 @@ def run_trajectory(lattice_size, conf_npy=None, rational_dir=None, i_conf=200000
     U = hmc.GaugeField(dims)
     U.from_host(lattice.tile_4x4x4x8(np.load(conf_npy), dims))
     rng.reset(i_conf)
+    if log:   # a line after each step of the outermost integrator level
+        hmcobj.integrator.on_step = lambda level, step: log(
+            f"  MD level {level}: step {step} done ({time.perf_counter() - t0:.1f} s)")
     dwf_amd.synchronize()
     t0 = time.perf_counter()
     out = hmcobj.update(U)

@@ -421,6 +421,7 @@ class Integrator_Omelyan:
     def __init__(self, update_p, update_U, level, Nstep, lambda_omelyan=0.1931833275):
         self.update_p, self.update_U = update_p, update_U
         self.level, self.Nstep, self.lam = int(level), int(Nstep), float(lambda_omelyan)
+        self.on_step = None   # optional callback(level, step) after each step
 
     def invalidate_cache(self):
         self.update_p.invalidate_cache()
@@ -437,6 +438,8 @@ class Integrator_Omelyan:
             self.update_U.evolve(e * 0.5, iP, U)
             if istep < self.Nstep:
                 self.update_p.evolve(e * self.lam * 2.0, iP, U)
+            if self.on_step:
+                self.on_step(self.level, istep)
         self.update_p.evolve(e * self.lam, iP, U)
 
 
