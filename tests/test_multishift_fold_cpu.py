@@ -5,16 +5,28 @@ the reference iteration (ashiftsolver_CG-tmpl.h:216-286: every shift updated and
 its |p_j|^2 computed every iteration), on a dense SPD matrix with condition
 number 1e4 and 12 shifts: the same iterations, the same truncation iteration of
 every shift, and the same solutions to round-off.  A planted mistake (the
-recurrence without the alpha_j^2 factor) changes the truncation."""
+recurrence without the alpha_j^2 factor) changes the truncation.
+
+With A = M^T M (D^dag D), the unshifted system's <p0, (A + sigma0) p0> taken as
+|M p0|^2 + sigma0 |p0|^2 (the first norm fused into the operator, the second
+into the p0 update) and sigma0 p0 added in the residual update is the same
+iteration up to rounding: on a problem that converges before CG loses
+orthogonality, the iteration count within 2, solutions equal to the solver's
+tolerance and every shift's true residual below it (a run past the loss of
+orthogonality, 709 iterations at n = 400, took 719: rounding, not a different
+method).  Dropping the sigma0 |p0|^2 term does not converge."""
 import numpy as np
 import pytest
 
 
-def _problem(n=400, seed=3):
+def _problem(n=400, seed=3, with_m=False):
     rng = np.random.default_rng(seed)
     q, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    a = (q * np.logspace(-3, 1, n)) @ q.T
-    return a, rng.standard_normal(n), np.logspace(-3, 1, 12)
+    lam = np.logspace(-3, 1, n)
+    a = (q * lam) @ q.T
+    m = (q * np.sqrt(lam)).T   # a = m^T m
+    out = (a, rng.standard_normal(n), np.logspace(-3, 1, 12))
+    return out + (m,) if with_m else out
 
 
 def reference(a, b, sig, stop):
@@ -57,8 +69,10 @@ def reference(a, b, sig, stop):
             return it, trunc, x
 
 
-def folded(a, b, sig, stop, K=16, planted=False):
-    """The same iteration with the shifts folded once per block of K iterations."""
+def folded(a, b, sig, stop, K=16, planted=False, m_op=None, planted_fused=False):
+    """The same iteration with the shifts folded once per block of K iterations;
+    with m_op (a = m_op^T m_op): pa = |m_op p0|^2 + sigma0 |p0|^2 and sigma0 p0
+    added in the residual update."""
     n = len(sig)
     X = [np.zeros_like(b) for _ in sig]
     P = [b.copy() for _ in sig]
@@ -70,15 +84,23 @@ def folded(a, b, sig, stop, K=16, planted=False):
     ca, cb = np.ones(n), np.zeros(n)
     cc, cd = np.zeros((n, K)), np.zeros((n, K))
     rr, m, nfold, trunc = b @ b, 0, n, {}
+    pp0 = b @ b   # |p0|^2
     for it in range(10000):
-        s = a @ p0 + sig[0] * p0
-        rrp, pa = rr, s @ p0
+        rrp = rr
+        if m_op is None:
+            s = a @ p0 + sig[0] * p0
+            pa = s @ p0
+        else:
+            u = m_op @ p0
+            s = m_op.T @ u + sig[0] * p0
+            pa = u @ u + (0.0 if planted_fused else sig[0]) * pp0
         beta = -rrp / pa
         x0 -= beta * p0
         rn = R[m] + beta * s
         rr = rn @ rn
         alpha = rr / rrp
         p0 = rn + alpha * p0
+        pp0 = p0 @ p0
         pp[0] = rr
         m += 1
         R.append(rn)
@@ -133,3 +155,26 @@ def test_planted_recurrence_mistake_changes_truncation():
     _, tr_r, _ = reference(a, b, sig, 1e-22)
     _, tr_p, _ = folded(a, b, sig, 1e-22, 16, planted=True)
     assert tr_p != tr_r
+
+
+def _problem_mtm(n=600, seed=3):
+    """A = M^T M with eigenvalues on [1e-2, 1] (CG converges in ~120 < n iterations)."""
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.logspace(-2, 0, n)
+    return (q * lam) @ q.T, rng.standard_normal(n), np.logspace(-3, 1, 12), (q * np.sqrt(lam)).T
+
+
+def test_fused_unshifted_reductions_match_the_reference_iteration():
+    a, b, sig, m = _problem_mtm()
+    it_r, _, x_r = reference(a, b, sig, 1e-22)
+    it_f, _, x_f = folded(a, b, sig, 1e-22, 16, m_op=m)
+    assert abs(it_f - it_r) <= 2
+    for s_, xr, xf in zip(sig, x_r, x_f):
+        assert np.linalg.norm(xf - xr) / np.linalg.norm(xr) < 1e-10
+        assert np.linalg.norm(a @ xf + s_ * xf - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_planted_missing_shift_term_in_pa_does_not_converge():
+    a, b, sig, m = _problem_mtm()
+    assert folded(a, b, sig, 1e-22, 16, m_op=m, planted_fused=True) is None
