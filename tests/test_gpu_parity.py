@@ -279,6 +279,30 @@ def test_multishift_matches_oracle():
         assert rel(x, xr) < 1e-8
 
 
+def test_multishift_ddagger_mode_matches_oracle():
+    """A mode other than DdagD (here DDdag) keeps the reference's form of the
+    unshifted update (s += sigma0 p0 and <p0, s> in one pass); DdagD takes
+    <p0, s> from |D p0|^2 fused into the operator."""
+    prm = ((4, 4, 4, 4), 6, 0.05, 1.6, 1.0, 0.0, (1, 1, 1, -1))
+    print("case multishift DDdag", prm, flush=True)
+    orc, gpu, U = pair(prm)
+    b = lattice.random_spinor5(prm[0], prm[1], 17)
+    sigma = lattice.log_spaced_shifts(6, 1e-3, 10.0)
+    xo, no, do, rc = orc.shift_cg(b, sigma, "DDdag", niter=3000, stop_cond2=1e-24)
+    gpu.set_mode("DDdag")
+    fb = gpu.new_field()
+    gpu.convert(fb, b)
+    xs = [gpu.new_field() for _ in sigma]
+    n, d = dwf_amd.AShiftsolver_CG(gpu, 3000, 1e-24).solve(xs, sigma, fb)
+    assert abs(n - no) <= 2
+    assert d < 1e-10
+    for s, fx, xr in zip(sigma, xs, xo):
+        x = gpu.reverse(fx)
+        r = orc.mult("DDdag", x) + s * x - b
+        assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+        assert rel(x, xr) < 1e-8
+
+
 def test_multishift_many_shifts():
     """More shifts than the one-pass update handles (32): the per-shift launch
     path; every shift meets the stopping condition."""
