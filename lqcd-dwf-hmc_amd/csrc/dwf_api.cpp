@@ -497,10 +497,32 @@ static int op_Ddag(dwf_fopr_s* op, void* v, const void* w, const void* pre = nul
   RC(run_hop(op, true, 0, v, t1, M_G2, w, 1.0));
   return DWF_OK;
 }
+// Ls = 16 and 12 (the matrix-core epilogue): u = D w need not be stored in
+// DdagD.  The last hop of D forms u for u2 = Di^T u (and a norm) without storing
+// it, and the last hop of Ddag adds u as Dee^T u2 through its second chiral
+// matrix, which saves one field write per DdagD.
+static bool u_unstored(const dwf_fopr_s* op) { return op->Ls == 16 || op->Ls == 12; }
+// Ddag u from u2 = Di^T u alone (its prologue already done, u not stored)
+static int op_Ddag_u2(dwf_fopr_s* op, void* v, const void* u2) {
+  void* t1;
+  RC(get_work(op, 1, &t1));
+  RC(run_hop(op, true, 1, t1, u2, M_G1));
+  Epi g;
+  g.E = mat(op, M_G2);
+  g.X = u2;
+  g.G = mat(op, M_DEET);
+  HIPCHK(launch_dwf(op->prec, K_HOPDAG, v, t1, op->links, op->nbr, 0, op->Vh, op->Ls, g,
+                    g_ctx.stream));
+  return DWF_OK;
+}
 static int op_DdagD(dwf_fopr_s* op, void* v, const void* w) {
   void *u, *u2;
-  RC(get_work(op, 2, &u));
   RC(get_work(op, 3, &u2));
+  if (u_unstored(op)) {
+    RC(op_D(op, nullptr, w, u2));
+    return op_Ddag_u2(op, v, u2);
+  }
+  RC(get_work(op, 2, &u));
   RC(op_D(op, u, w, u2));
   RC(op_Ddag(op, v, u, u2));
   return DWF_OK;
@@ -532,9 +554,10 @@ static int op_DdagD_nrm(dwf_fopr_s* op, void* v, const void* w, double* nrm,
   e.F = mat(op, M_DIT);
   e.nrm_partial = op->hop_partial;
   e.nrm_result = nrm;
-  HIPCHK(launch_dwf(op->prec, K_HOP, u, t1, op->links, op->nbr, 0, op->Vh, op->Ls, e,
-                    g_ctx.stream));
-  return op_Ddag(op, v, u, u2);
+  const bool keep_u = !u_unstored(op);
+  HIPCHK(launch_dwf(op->prec, K_HOP, keep_u ? u : nullptr, t1, op->links, op->nbr, 0, op->Vh,
+                    op->Ls, e, g_ctx.stream));
+  return keep_u ? op_Ddag(op, v, u, u2) : op_Ddag_u2(op, v, u2);
 }
 // One CG step's operator part with the residual update fused into the last
 // hop (Ls = 16): mpw = Mp p (non-spin-scalar Mp) or p itself (Mp folded into
