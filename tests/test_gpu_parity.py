@@ -23,9 +23,12 @@ PARAMS = [
 ]
 LS16 = [((4, 4, 4, 8), 16, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1)),      # matrix-core epilogue
         ((4, 2, 4, 4), 16, 0.05, 1.0, 1.5, 0.5, (1, 1, 1, -1))]
-# Ls = 12 (the reference benchmark's Ls): compile-time Ls with a partial last
-# wave per block (42 sites x 6 threads at 16-bit storage)
+# Ls = 12 (the reference benchmark's Ls): compile-time Ls; a 16-bit site has 8
+# threads (6 used) and an FP32 site 16 (12 used)
 LS12 = [((4, 4, 4, 8), 12, 0.02, 1.6, 1.5, 0.5, (1, 1, 1, -1))]
+# the same with a partial last block: Vh = 216 half sites (blocks of 32 sites at
+# FP32 and 16-bit storage, 16 at FP64)
+LS12_PARTIAL = [((6, 6, 2, 6), 12, 0.04, 1.6, 1.0, 0.0, (1, 1, 1, -1))]
 MODES = ["D", "Ddag", "DdagD", "DDdag", "H", "Hdag", "Deo", "Doe", "Dee", "Doo",
          "Dee_inv", "Doo_inv"]
 
@@ -44,7 +47,7 @@ def pair(prm, precision="double", seed=5, eps=0.5):
     return orc, gpu, U
 
 
-@pytest.mark.parametrize("prm", PARAMS + LS16[1:] + LS12)
+@pytest.mark.parametrize("prm", PARAMS + LS16[1:] + LS12 + LS12_PARTIAL)
 def test_modes_fp64(prm):
     print("case", prm, flush=True)
     orc, gpu, U = pair(prm)
@@ -56,7 +59,7 @@ def test_modes_fp64(prm):
             assert rel(got, want) < 1e-13, (mode, dag, rel(got, want))
 
 
-@pytest.mark.parametrize("prm", PARAMS[:2] + LS16 + LS12)
+@pytest.mark.parametrize("prm", PARAMS[:2] + LS16 + LS12 + LS12_PARTIAL)
 @pytest.mark.parametrize("precision,tol", [("float", 2e-6), ("half", 4e-3), ("bfloat16", 3e-2)])
 def test_modes_reduced_precision(prm, precision, tol):
     orc, gpu, U = pair(prm, precision)
@@ -209,8 +212,10 @@ def test_richardson_mixed_precision_matches_oracle():
 
 
 @pytest.mark.parametrize("inner", ["half", "float", "bfloat16", "double"])
-@pytest.mark.parametrize("prm", LS16 + LS12 + [((4, 4, 4, 8), 12, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1))],
-                         ids=["ls16-shamir", "ls16-mobius", "ls12-mobius", "ls12-shamir"])
+@pytest.mark.parametrize("prm", LS16 + LS12 + [((4, 4, 4, 8), 12, 0.02, 1.6, 1.0, 0.0, (1, 1, 1, -1))]
+                         + LS12_PARTIAL,
+                         ids=["ls16-shamir", "ls16-mobius", "ls12-mobius", "ls12-shamir",
+                              "ls12-partial-block"])
 def test_richardson_low_precision_inner_matrix_core(prm, inner):
     """Mixed solve with the sync-free inner CG at Ls = 16 and 12 (matrix-core
     epilogue, fused p update + Mp prologue): the FP64 residual reaches the
