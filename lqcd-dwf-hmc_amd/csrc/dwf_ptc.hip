@@ -25,8 +25,11 @@ namespace dwf {
 
 constexpr int kPtcPaths = 11;
 
-__device__ __forceinline__ float bf16_to_f(unsigned short h) {
-  return __uint_as_float((unsigned)h << 16);
+// (re, im) += (ar + i ai)(br + i bi), two fused multiply-adds per part
+__device__ __forceinline__ void cmac(float& re, float& im, float ar, float ai, float br,
+                                     float bi) {
+  re = fmaf(-ai, bi, fmaf(ar, br, re));
+  im = fmaf(ai, br, fmaf(ar, bi, im));
 }
 
 __device__ __forceinline__ int ptc_rev(int q) {
@@ -67,10 +70,8 @@ __device__ __forceinline__ bool ptc_transport(float (&t)[24], const float* __res
       float re = 0, im = 0;
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        const float ur = u[2 * (b + 3 * a)], ui = u[2 * (b + 3 * a) + 1];
-        const float wr = w[2 * (b + 3 * d)], wi = w[2 * (b + 3 * d) + 1];
-        re += ur * wr - ui * wi;
-        im += ur * wi + ui * wr;
+        cmac(re, im, u[2 * (b + 3 * a)], u[2 * (b + 3 * a) + 1], w[2 * (b + 3 * d)],
+             w[2 * (b + 3 * d) + 1]);
       }
       t[2 * (a + 3 * d)] = re;
       t[2 * (a + 3 * d) + 1] = im;
@@ -78,74 +79,92 @@ __device__ __forceinline__ bool ptc_transport(float (&t)[24], const float* __res
   return true;
 }
 
-// One layer (ADJ = false) or its adjoint (ADJ = true): thread per
-// (parity, site, output channel, s), s fastest.  W: bf16 [q][a][b][alpha][beta][ri]
-// of the forward layer (C_out x C_in channels); for ADJ the kernel's
-// input channels are the forward outputs and out[b] = sum W^+ T_rev(q)(in[a]).
-template <bool ADJ>
+// One layer (ADJ = false) or its adjoint (ADJ = true).  Thread per (parity,
+// site, s), s fastest; block row blockIdx.y owns the AG output channels
+// a0 .. a0 + AG - 1, so each transport T_q(in[b]) is formed once per thread and
+// applied to AG output channels, and the weight addresses are uniform over the
+// block (scalar loads).  W: bf16 [q][a][b][alpha][beta][ri] of the forward layer
+// (C_out x C_in channels), read as one 32-bit word per complex entry (re low,
+// im high); for ADJ the kernel's input channels are the forward outputs and
+// out[b] = sum W^+ T_rev(q)(in[a]).  Each output channel's sum runs over q, then
+// b, as in the per-channel form.
+template <bool ADJ, int AG>
 __global__ void __launch_bounds__(256)
-k_ptc_layer(float* __restrict__ out, const float* __restrict__ in, int cin, int cout,
-            const unsigned short* __restrict__ W, int wcin, int wcout,
+k_ptc_layer(float* __restrict__ out, const float* __restrict__ in, int cin,
+            const unsigned* __restrict__ W, int wcin, int wcout,
             const float* __restrict__ links, const int* __restrict__ nbr, int Vh, int Ls) {
   const size_t fsz = (size_t)Vh * 24 * Ls;
-  const size_t total = (size_t)2 * Vh * cout * Ls;
+  const size_t total = (size_t)2 * Vh * Ls;
+  const int a0 = blockIdx.y * AG;
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
        idx += (size_t)gridDim.x * blockDim.x) {
     const int s = (int)(idx % Ls);
-    size_t r = idx / Ls;
-    const int a = (int)(r % cout);
-    r /= cout;
+    const size_t r = idx / Ls;
     const int site = (int)(r % Vh);
     const int p = (int)(r / Vh);
-    float acc[24];
+    float acc[AG][24];
 #pragma unroll
-    for (int k = 0; k < 24; ++k) acc[k] = 0;
+    for (int j = 0; j < AG; ++j)
+#pragma unroll
+      for (int k = 0; k < 24; ++k) acc[j][k] = 0;
     for (int q = 0; q < kPtcPaths; ++q) {
       const int qw = ADJ ? ptc_rev(q) : q;   // weight path
       for (int b = 0; b < cin; ++b) {
         float t[24];
         if (!ptc_transport(t, in + (size_t)b * 2 * fsz, links, nbr, q, p, site, s, Vh, Ls))
           continue;
-        // forward: W[qw][a][b];  adjoint: (W[qw][b][a])^+
-        const int wa = ADJ ? b : a, wb = ADJ ? a : b;
-        const unsigned short* w = W + (((size_t)qw * wcout + wa) * wcin + wb) * 32;
 #pragma unroll
-        for (int al = 0; al < 4; ++al)
+        for (int j = 0; j < AG; ++j) {
+          // forward: W[qw][a][b];  adjoint: (W[qw][b][a])^+
+          const int a = a0 + j;
+          const int wa = ADJ ? b : a, wb = ADJ ? a : b;
+          const unsigned* w = W + (((size_t)qw * wcout + wa) * wcin + wb) * 16;
 #pragma unroll
-          for (int be = 0; be < 4; ++be) {
-            // element (al, be) of the applied spin matrix
-            const int i = ADJ ? (be * 4 + al) : (al * 4 + be);
-            const float wr = bf16_to_f(w[2 * i]);
-            const float wi = ADJ ? -bf16_to_f(w[2 * i + 1]) : bf16_to_f(w[2 * i + 1]);
+          for (int al = 0; al < 4; ++al)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              const float tr = t[2 * (c + 3 * be)], ti = t[2 * (c + 3 * be) + 1];
-              acc[2 * (c + 3 * al)] += wr * tr - wi * ti;
-              acc[2 * (c + 3 * al) + 1] += wr * ti + wi * tr;
+            for (int be = 0; be < 4; ++be) {
+              // element (al, be) of the applied spin matrix
+              const unsigned e = w[ADJ ? (be * 4 + al) : (al * 4 + be)];
+              const float wr = __uint_as_float(e << 16);
+              const float wi = ADJ ? -__uint_as_float(e & 0xffff0000u)
+                                   : __uint_as_float(e & 0xffff0000u);
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                cmac(acc[j][2 * (c + 3 * al)], acc[j][2 * (c + 3 * al) + 1], wr, wi,
+                     t[2 * (c + 3 * be)], t[2 * (c + 3 * be) + 1]);
+              }
             }
-          }
+        }
       }
     }
-    float* o = out + (size_t)a * 2 * fsz + (size_t)p * fsz + (size_t)site * 24 * Ls + s;
 #pragma unroll
-    for (int k = 0; k < 24; ++k) o[k * Ls] = acc[k];
+    for (int j = 0; j < AG; ++j) {
+      float* o = out + (size_t)(a0 + j) * 2 * fsz + (size_t)p * fsz + (size_t)site * 24 * Ls + s;
+#pragma unroll
+      for (int k = 0; k < 24; ++k) o[k * Ls] = acc[j][k];
+    }
   }
 }
 
-// Weight gradient of one layer: block (x, y = q * cin + b, z = a) adds, over
-// its share of the lattice, G[q][a][b][al][be] = sum_{x, s, c}
-// g[a](al, c) conj(T_q(in[b])(be, c)); block partials [y][z][block][32].
+// Weight gradient of one layer: block (x, y = q * cin + b, z) adds, over its
+// share of the lattice, G[q][a][b][al][be] = sum_{x, s, c}
+// g[a](al, c) conj(T_q(in[b])(be, c)) for the AG output channels
+// a = z AG .. z AG + AG - 1 (the transport formed once for all of them);
+// block partials [y][a][block][32].
+template <int AG>
 __global__ void __launch_bounds__(256)
 k_ptc_wgrad(double* __restrict__ partial, const float* __restrict__ g,
             const float* __restrict__ in, int cin, int cout, const float* __restrict__ links,
             const int* __restrict__ nbr, int Vh, int Ls) {
   const int q = blockIdx.y / cin, b = blockIdx.y - (blockIdx.y / cin) * cin;
-  const int a = blockIdx.z;
+  const int a0 = blockIdx.z * AG;
   const size_t fsz = (size_t)Vh * 24 * Ls;
   const size_t total = (size_t)2 * Vh * Ls;
-  float acc[32];
+  float acc[AG][32];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) acc[k] = 0;
+  for (int j = 0; j < AG; ++j)
+#pragma unroll
+    for (int k = 0; k < 32; ++k) acc[j][k] = 0;
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
        idx += (size_t)gridDim.x * blockDim.x) {
     const int s = (int)(idx % Ls);
@@ -154,37 +173,41 @@ k_ptc_wgrad(double* __restrict__ partial, const float* __restrict__ g,
     const int p = (int)(r / Vh);
     float t[24];
     if (!ptc_transport(t, in + (size_t)b * 2 * fsz, links, nbr, q, p, site, s, Vh, Ls)) continue;
-    const float* gp = g + (size_t)a * 2 * fsz + (size_t)p * fsz + (size_t)site * 24 * Ls + s;
-    float gv[24];
 #pragma unroll
-    for (int k = 0; k < 24; ++k) gv[k] = gp[k * Ls];
+    for (int j = 0; j < AG; ++j) {
+      const float* gp =
+          g + (size_t)(a0 + j) * 2 * fsz + (size_t)p * fsz + (size_t)site * 24 * Ls + s;
+      float gv[24];
 #pragma unroll
-    for (int al = 0; al < 4; ++al)
+      for (int k = 0; k < 24; ++k) gv[k] = gp[k * Ls];
 #pragma unroll
-      for (int be = 0; be < 4; ++be)
+      for (int al = 0; al < 4; ++al)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const float gr = gv[2 * (c + 3 * al)], gi = gv[2 * (c + 3 * al) + 1];
-          const float tr = t[2 * (c + 3 * be)], ti = t[2 * (c + 3 * be) + 1];
-          acc[2 * (al * 4 + be)] += gr * tr + gi * ti;
-          acc[2 * (al * 4 + be) + 1] += gi * tr - gr * ti;
-        }
+        for (int be = 0; be < 4; ++be)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            // g conj(t)
+            cmac(acc[j][2 * (al * 4 + be)], acc[j][2 * (al * 4 + be) + 1], gv[2 * (c + 3 * al)],
+                 gv[2 * (c + 3 * al) + 1], t[2 * (c + 3 * be)], -t[2 * (c + 3 * be) + 1]);
+    }
   }
-  __shared__ double sh[4][32];
+  __shared__ double sh[4][AG * 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    double v = acc[k];
+  for (int j = 0; j < AG; ++j)
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) sh[wv][k] = v;
-  }
+    for (int k = 0; k < 32; ++k) {
+      double v = acc[j][k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) sh[wv][j * 32 + k] = v;
+    }
   __syncthreads();
-  if (threadIdx.x < 32) {
+  if (threadIdx.x < AG * 32) {
+    const int j = threadIdx.x >> 5, k = threadIdx.x & 31;
     const double v = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] +
                      sh[3][threadIdx.x];
-    partial[(((size_t)blockIdx.y * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x) * 32 +
-            threadIdx.x] = v;
+    partial[(((size_t)blockIdx.y * cout + a0 + j) * gridDim.x + blockIdx.x) * 32 + k] = v;
   }
 }
 
@@ -238,25 +261,56 @@ static unsigned ptc_grid(size_t n) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
+template <bool ADJ, int AG>
+static void ptc_layer_launch(float* out, const float* in, int cin, int cout, const unsigned* W,
+                             int wcin, int wcout, const float* links, const int* nbr, int Vh,
+                             int Ls, hipStream_t st) {
+  const dim3 grid(ptc_grid((size_t)2 * Vh * Ls), cout / AG);
+  hipLaunchKernelGGL((k_ptc_layer<ADJ, AG>), grid, dim3(256), 0, st, out, in, cin, W, wcin, wcout,
+                     links, nbr, Vh, Ls);
+}
+
+template <bool ADJ>
+static void ptc_layer_dispatch(float* out, const float* in, int cin, int cout, const unsigned* W,
+                               int wcin, int wcout, const float* links, const int* nbr, int Vh,
+                               int Ls, hipStream_t st) {
+  // four output channels per thread (227 registers, two waves per SIMD); eight
+  // need 324 registers, one wave per SIMD, and are slower
+  if (cout % 4 == 0)
+    ptc_layer_launch<ADJ, 4>(out, in, cin, cout, W, wcin, wcout, links, nbr, Vh, Ls, st);
+  else if (cout % 2 == 0)
+    ptc_layer_launch<ADJ, 2>(out, in, cin, cout, W, wcin, wcout, links, nbr, Vh, Ls, st);
+  else
+    ptc_layer_launch<ADJ, 1>(out, in, cin, cout, W, wcin, wcout, links, nbr, Vh, Ls, st);
+}
+
 hipError_t launch_ptc_layer(bool adj, float* out, const float* in, int cin, int cout,
                             const unsigned short* W, int wcin, int wcout, const float* links,
                             const int* nbr, int Vh, int Ls, hipStream_t st) {
-  const size_t total = (size_t)2 * Vh * cout * Ls;
+  // each complex weight is one aligned 32-bit word (re, im)
+  if (reinterpret_cast<uintptr_t>(W) % 4 != 0 || cout < 1) return hipErrorInvalidValue;
+  const unsigned* w = reinterpret_cast<const unsigned*>(W);
   if (adj)
-    hipLaunchKernelGGL(k_ptc_layer<true>, dim3(ptc_grid(total)), dim3(256), 0, st, out, in, cin,
-                       cout, W, wcin, wcout, links, nbr, Vh, Ls);
+    ptc_layer_dispatch<true>(out, in, cin, cout, w, wcin, wcout, links, nbr, Vh, Ls, st);
   else
-    hipLaunchKernelGGL(k_ptc_layer<false>, dim3(ptc_grid(total)), dim3(256), 0, st, out, in, cin,
-                       cout, W, wcin, wcout, links, nbr, Vh, Ls);
+    ptc_layer_dispatch<false>(out, in, cin, cout, w, wcin, wcout, links, nbr, Vh, Ls, st);
   return hipGetLastError();
 }
 
 hipError_t launch_ptc_wgrad(double* G, double* partial, int nblk, const float* g, const float* in,
                             int cin, int cout, const float* links, const int* nbr, int Vh, int Ls,
                             hipStream_t st) {
-  const dim3 grid(nblk, kPtcPaths * cin, cout);
-  hipLaunchKernelGGL(k_ptc_wgrad, grid, dim3(256), 0, st, partial, g, in, cin, cout, links, nbr,
-                     Vh, Ls);
+  if (cout < 1) return hipErrorInvalidValue;
+  // two output channels per thread: four need 128 accumulators (266 registers,
+  // one wave per SIMD), which is slower
+  const int ag = cout % 2 == 0 ? 2 : 1;
+  const dim3 grid(nblk, kPtcPaths * cin, cout / ag);
+  if (ag == 2)
+    hipLaunchKernelGGL(k_ptc_wgrad<2>, grid, dim3(256), 0, st, partial, g, in, cin, cout, links,
+                       nbr, Vh, Ls);
+  else
+    hipLaunchKernelGGL(k_ptc_wgrad<1>, grid, dim3(256), 0, st, partial, g, in, cin, cout, links,
+                       nbr, Vh, Ls);
   const int nyz = kPtcPaths * cin * cout;
   hipLaunchKernelGGL(k_ptc_wgrad_finish, dim3(nyz), dim3(64), 0, st, G, partial, nblk, nyz);
   return hipGetLastError();

@@ -4,13 +4,16 @@ of known size, the calibration for hardware byte counters).
     python3 lqcd-dwf-hmc_amd/dwf_amd/kernel_bench.py [--mode Deo] [--prec double]
                                                      [--reps 50] [--dims 16 16 16 32] [--Ls 16]
                                                      [--lpc-train DEGREE]
+                                                     [--ptc 1 8 8 8 1]
 
 Runs <reps> launches of the operator mode (and <reps> field copies) on the
 library stream, timed with HIP events, and prints one line per measurement.
 The workload is the benchmark lattice: links exp(0.3 i H), Shamir b = 1, c = 0,
 mq = 0.01, M0 = 1.6, antiperiodic in t.  --lpc-train also trains the learned
 polynomial preconditioner of that degree once on the operator (its Gram
-matrices run on the matrix cores for an FP32 operator at Ls <= 16).
+matrices run on the matrix cores for an FP32 operator at Ls <= 16).  --ptc
+CHANNELS also times one application of the parallel-transport network with
+those channel counts (FP32 operator; identity weights plus noise 0.05).
 """
 from __future__ import annotations
 
@@ -61,6 +64,7 @@ def main():
     ap.add_argument("--dims", type=int, nargs=4, default=[16, 16, 16, 32])
     ap.add_argument("--Ls", type=int, default=16)
     ap.add_argument("--lpc-train", type=int, default=0)
+    ap.add_argument("--ptc", type=int, nargs="+", default=None)
     a = ap.parse_args()
     dims = tuple(a.dims)
     op = dwf_amd.AFopr_Domainwall_5din_eo(dims, a.Ls, 0.01, 1.6, 1.0, 0.0, (1, 1, 1, -1), a.prec)
@@ -84,6 +88,12 @@ def main():
         loss = lpc.train(op, nvec=2)
         dwf_amd.synchronize()
         print(f"learned polynomial degree {a.lpc_train} trained: loss {loss:.4f}", flush=True)
+    if a.ptc:
+        from dwf_amd.ptc import ParallelTransportNet
+        net = ParallelTransportNet(op, a.ptc)
+        net.set_weights(net.identity_weights(scale=0.5, noise=0.05, seed=1))
+        us = ev.time(lambda: net.apply(v, w), a.reps, st)
+        print(f"network {tuple(a.ptc)}: {us / 1e3:.2f} ms/application", flush=True)
     us = ev.time(lambda: v.copy_from(w), a.reps, st)
     print(f"copy of one field ({field_bytes} bytes read, {field_bytes} written): "
           f"{us:.2f} us/launch, {2 * field_bytes / us / 1e3:.0f} GB/s", flush=True)
