@@ -146,6 +146,152 @@ k_ptc_layer(float* __restrict__ out, const float* __restrict__ in, int cin,
   }
 }
 
+// Matrix-core form of the same layer for an even number of output channels:
+// per path q and input channel b the contraction is the real product
+//   [out_re; out_im](a, alpha) += [Wr -Wi; Wi Wr](a, alpha; beta) [t_re; t_im](beta)
+// for every colour, computed as v_mfma_f32_16x16x4f32 tiles D(16x16) +=
+// A(16x4) B(4x16) with
+//   D rows  m = 8 a' + 2 alpha + re/im  (NMT tiles of two output channels),
+//   K       beta (4 spins), one product for t_re and one for t_im,
+//   columns 16 points (site, s) of one parity, consecutive in site * Ls + s.
+// Lane l holds A[l % 16][l / 16] and B[l / 16][l % 16], so lane l computes
+// the transport of spin beta = l / 16 at point l % 16 itself (3 colours, 36
+// multiply-adds for a hop path) and holds the weight of row l % 16, column
+// beta; register r of its result holds D[4 (l / 16) + r][l % 16].  One wave
+// per 16 points, all output channels; each output element is a sum over
+// (q, b) of 4-spin matrix-core products in FP32.
+template <bool ADJ, int NMT>
+__global__ void __launch_bounds__(256)
+k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin,
+                 const unsigned* __restrict__ W, int wcin, int wcout,
+                 const float* __restrict__ links, const int* __restrict__ nbr, int Vh, int Ls) {
+  const int lane = threadIdx.x & 63;
+  const int n = lane & 15, g = lane >> 4;
+  const size_t fsz = (size_t)Vh * 24 * Ls;
+  const int npts = Vh * Ls;            // points of one parity
+  const int tpp = (npts + 15) / 16;    // tiles of one parity
+  const int wpb = blockDim.x >> 6;
+  // the A row of this lane: channel 2 mt + am, spin row al, re (0) / im (1)
+  const int am = n >> 3, al = (n >> 1) & 3, ro = n & 1;
+  for (int tile = blockIdx.x * wpb + (threadIdx.x >> 6); tile < 2 * tpp;
+       tile += gridDim.x * wpb) {
+    const int p = tile / tpp;
+    const int P = (tile - p * tpp) * 16 + n;
+    const bool valid = P < npts;
+    const int Pc = valid ? P : npts - 1;
+    const int site = Pc / Ls, s = Pc - (Pc / Ls) * Ls;
+    flt4_t acc[NMT][3];
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[mt][c] = flt4_t{0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < kPtcPaths; ++q) {
+      const int qw = ADJ ? ptc_rev(q) : q;   // weight path
+      const bool hop = q >= 1 && q <= 8;
+      int sp = p, ssite = site, ss = s;
+      bool ok = valid;
+      float u[18];
+      if (hop) {
+        const int dir = q - 1;
+        sp = 1 - p;
+        ssite = nbr[((size_t)p * 8 + dir) * Vh + site];
+        const float* L = links + ((size_t)p * 8 + dir) * 18 * Vh + site;
+#pragma unroll
+        for (int e = 0; e < 18; ++e) u[e] = L[(size_t)e * Vh];
+      } else if (q >= 9) {
+        ss = q == 9 ? s + 1 : s - 1;
+        ok = valid && ss >= 0 && ss < Ls;
+      }
+      // the spinor of input channel b at the source point and this lane's
+      // weight words; loaded one channel ahead of the matrix-core products
+      const float* src0 =
+          in + (size_t)sp * fsz + (size_t)ssite * 24 * Ls + (ok ? ss : 0) + (size_t)6 * g * Ls;
+      auto load = [&](int b, float (&w)[6], unsigned (&e)[NMT]) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = 0.f;
+        if (ok) {
+          const float* src = src0 + (size_t)b * 2 * fsz;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) w[k] = src[(size_t)k * Ls];
+        }
+        // element (al, beta = g) of the applied spin matrix of channel
+        // a = 2 mt + am: forward W[qw][a][b], adjoint (W[qw][b][a])^+
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) {
+          const int a = 2 * mt + am;
+          const int wa = ADJ ? b : a, wb = ADJ ? a : b;
+          e[mt] = W[(((size_t)qw * wcout + wa) * wcin + wb) * 16 + (ADJ ? g * 4 + al : al * 4 + g)];
+        }
+      };
+      float wn[6];
+      unsigned en[NMT];
+      load(0, wn, en);
+      for (int b = 0; b < cin; ++b) {
+        float w[6];
+        unsigned e[NMT];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = wn[k];
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) e[mt] = en[mt];
+        if (b + 1 < cin) load(b + 1, wn, en);
+        float ar[NMT], ai[NMT];
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) {
+          const float wr = __uint_as_float(e[mt] << 16);
+          const float wi = ADJ ? -__uint_as_float(e[mt] & 0xffff0000u)
+                               : __uint_as_float(e[mt] & 0xffff0000u);
+          ar[mt] = ro ? wi : wr;    // multiplies t_re
+          ai[mt] = ro ? wr : -wi;   // multiplies t_im
+        }
+        // B operand: the transported colour vector of spin g at this point
+        float t[6];
+        if (hop) {
+#pragma unroll
+          for (int ca = 0; ca < 3; ++ca) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int cb = 0; cb < 3; ++cb)
+              cmac(re, im, u[2 * (cb + 3 * ca)], u[2 * (cb + 3 * ca) + 1], w[2 * cb],
+                   w[2 * cb + 1]);
+            t[2 * ca] = re;
+            t[2 * ca + 1] = im;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) t[k] = w[k];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int mt = 0; mt < NMT; ++mt)
+            acc[mt][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[mt], t[2 * c], acc[mt][c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int mt = 0; mt < NMT; ++mt)
+            acc[mt][c] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(ai[mt], t[2 * c + 1], acc[mt][c], 0, 0, 0);
+      }
+    }
+    if (valid) {
+      // register r: row 4 g + r = channel 2 mt + (g >> 1), spin 2 (g & 1) + (r >> 1),
+      // re / im r & 1
+#pragma unroll
+      for (int mt = 0; mt < NMT; ++mt) {
+        float* o = out + (size_t)(2 * mt + (g >> 1)) * 2 * fsz + (size_t)p * fsz +
+                   (size_t)site * 24 * Ls + s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int alo = 2 * (g & 1) + (r >> 1);
+            o[(size_t)((r & 1) + 2 * (c + 3 * alo)) * Ls] = acc[mt][c][r];
+          }
+      }
+    }
+  }
+}
+
 // Weight gradient of one layer: block (x, y = q * cin + b, z) adds, over its
 // share of the lattice, G[q][a][b][al][be] = sum_{x, s, c}
 // g[a](al, c) conj(T_q(in[b])(be, c)) for the AG output channels
@@ -270,12 +416,36 @@ static void ptc_layer_launch(float* out, const float* in, int cin, int cout, con
                      links, nbr, Vh, Ls);
 }
 
+template <bool ADJ, int NMT>
+static void ptc_layer_mfma_launch(float* out, const float* in, int cin, const unsigned* W,
+                                  int wcin, int wcout, const float* links, const int* nbr, int Vh,
+                                  int Ls, hipStream_t st) {
+  const size_t tiles = 2 * (((size_t)Vh * Ls + 15) / 16);
+  size_t blocks = (tiles + 3) / 4;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL((k_ptc_layer_mfma<ADJ, NMT>), dim3((unsigned)blocks), dim3(256), 0, st, out,
+                     in, cin, W, wcin, wcout, links, nbr, Vh, Ls);
+}
+
 template <bool ADJ>
 static void ptc_layer_dispatch(float* out, const float* in, int cin, int cout, const unsigned* W,
                                int wcin, int wcout, const float* links, const int* nbr, int Vh,
                                int Ls, hipStream_t st) {
-  // four output channels per thread (227 registers, two waves per SIMD); eight
-  // need 324 registers, one wave per SIMD, and are slower
+  // an even number of output channels: the matrix-core form, all channels of 16
+  // points per wave
+  switch (cout) {
+    case 2: return ptc_layer_mfma_launch<ADJ, 1>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 4: return ptc_layer_mfma_launch<ADJ, 2>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 6: return ptc_layer_mfma_launch<ADJ, 3>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 8: return ptc_layer_mfma_launch<ADJ, 4>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 10: return ptc_layer_mfma_launch<ADJ, 5>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 12: return ptc_layer_mfma_launch<ADJ, 6>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 14: return ptc_layer_mfma_launch<ADJ, 7>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    case 16: return ptc_layer_mfma_launch<ADJ, 8>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
+    default: break;
+  }
+  // otherwise the VALU form: four output channels per thread (227 registers, two
+  // waves per SIMD); eight need 324 registers, one wave per SIMD, and are slower
   if (cout % 4 == 0)
     ptc_layer_launch<ADJ, 4>(out, in, cin, cout, W, wcin, wcout, links, nbr, Vh, Ls, st);
   else if (cout % 2 == 0)
