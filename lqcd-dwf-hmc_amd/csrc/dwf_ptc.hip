@@ -146,6 +146,22 @@ k_ptc_layer(float* __restrict__ out, const float* __restrict__ in, int cin,
   }
 }
 
+// Raw buffer resource over nbytes at base (gfx9 descriptor word 3): loads at
+// byte offsets >= nbytes return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ptc_rsrc(const void* base, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)nbytes, 0x00020000);
+}
+// the b32 load returns the 32 loaded bits as an integer: reinterpret, never convert
+__device__ __forceinline__ unsigned ptc_ldu(__amdgpu_buffer_rsrc_t r, unsigned voff,
+                                            unsigned soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+  static_assert(sizeof(v) == 4, "32-bit buffer load");
+  return __builtin_bit_cast(unsigned, v);
+}
+__device__ __forceinline__ float ptc_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __uint_as_float(ptc_ldu(r, voff, soff));
+}
+
 // Matrix-core form of the same layer for an even number of output channels:
 // per path q and input channel b the contraction is the real product
 //   [out_re; out_im](a, alpha) += [Wr -Wi; Wi Wr](a, alpha; beta) [t_re; t_im](beta)
@@ -160,19 +176,32 @@ k_ptc_layer(float* __restrict__ out, const float* __restrict__ in, int cin,
 // beta; register r of its result holds D[4 (l / 16) + r][l % 16].  One wave
 // per 16 points, all output channels; each output element is a sum over
 // (q, b) of 4-spin matrix-core products in FP32.
+// three waves per SIMD (at most 168 registers) fit without spills up to NMT = 5
 template <bool ADJ, int NMT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, NMT <= 5 ? 3 : 2)
 k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin,
                  const unsigned* __restrict__ W, int wcin, int wcout,
                  const float* __restrict__ links, const int* __restrict__ nbr, int Vh, int Ls) {
+  // loads go through buffer resources with 32-bit byte offsets: per lane the
+  // point / spin part, per wave (SGPR) the channel and element part; a lane
+  // whose source is outside the lattice or the fifth dimension reads at
+  // kOut and gets 0.  The launcher checks that every offset fits.
+  constexpr unsigned kOut = 0x80000000u;
   const int lane = threadIdx.x & 63;
   const int n = lane & 15, g = lane >> 4;
   const size_t fsz = (size_t)Vh * 24 * Ls;
   const int npts = Vh * Ls;            // points of one parity
   const int tpp = (npts + 15) / 16;    // tiles of one parity
   const int wpb = blockDim.x >> 6;
-  // the A row of this lane: channel 2 mt + am, spin row al, re (0) / im (1)
+  const unsigned lsb = 4u * (unsigned)Ls, vhb = 4u * (unsigned)Vh;
+  const __amdgpu_buffer_rsrc_t rl = ptc_rsrc(links, (unsigned)(2 * 8 * 18) * vhb);
+  const __amdgpu_buffer_rsrc_t rw =
+      ptc_rsrc(W, 4u * (unsigned)(kPtcPaths * wcin * wcout * 16));
+  // the A row of this lane: channel 2 mt + am, spin row al, re (0) / im (1);
+  // its weight word within the (q, b, mt) block
   const int am = n >> 3, al = (n >> 1) & 3, ro = n & 1;
+  const unsigned wlane =
+      4u * (ADJ ? (unsigned)(am * 16 + g * 4 + al) : (unsigned)(am * wcin * 16 + al * 4 + g));
   for (int tile = blockIdx.x * wpb + (threadIdx.x >> 6); tile < 2 * tpp;
        tile += gridDim.x * wpb) {
     const int p = tile / tpp;
@@ -188,39 +217,34 @@ k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin,
     for (int q = 0; q < kPtcPaths; ++q) {
       const int qw = ADJ ? ptc_rev(q) : q;   // weight path
       const bool hop = q >= 1 && q <= 8;
-      int sp = p, ssite = site, ss = s;
-      bool ok = valid;
       float u[18];
+      unsigned woff;   // byte offset of spin g of the source point in a channel
       if (hop) {
         const int dir = q - 1;
-        sp = 1 - p;
-        ssite = nbr[((size_t)p * 8 + dir) * Vh + site];
-        const float* L = links + ((size_t)p * 8 + dir) * 18 * Vh + site;
+        const int nb = nbr[((size_t)p * 8 + dir) * Vh + site];
+        const unsigned loff = 4u * (unsigned)(((p * 8 + dir) * 18) * Vh + site);
 #pragma unroll
-        for (int e = 0; e < 18; ++e) u[e] = L[(size_t)e * Vh];
-      } else if (q >= 9) {
-        ss = q == 9 ? s + 1 : s - 1;
-        ok = valid && ss >= 0 && ss < Ls;
+        for (int e = 0; e < 18; ++e) u[e] = ptc_ld(rl, loff, (unsigned)e * vhb);
+        woff = valid ? 4u * (unsigned)((1 - p) * fsz + ((size_t)nb * 24 + 6 * g) * Ls + s) : kOut;
+      } else {
+        const int ss = q == 0 ? s : (q == 9 ? s + 1 : s - 1);
+        woff = valid && ss >= 0 && ss < Ls
+                   ? 4u * (unsigned)(p * fsz + ((size_t)site * 24 + 6 * g) * Ls + ss)
+                   : kOut;
       }
       // the spinor of input channel b at the source point and this lane's
       // weight words; loaded one channel ahead of the matrix-core products
-      const float* src0 =
-          in + (size_t)sp * fsz + (size_t)ssite * 24 * Ls + (ok ? ss : 0) + (size_t)6 * g * Ls;
       auto load = [&](int b, float (&w)[6], unsigned (&e)[NMT]) {
+        const __amdgpu_buffer_rsrc_t ri = ptc_rsrc(in + (size_t)b * 2 * fsz, 8u * (unsigned)fsz);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) w[k] = 0.f;
-        if (ok) {
-          const float* src = src0 + (size_t)b * 2 * fsz;
-#pragma unroll
-          for (int k = 0; k < 6; ++k) w[k] = src[(size_t)k * Ls];
-        }
+        for (int k = 0; k < 6; ++k) w[k] = ptc_ld(ri, woff, (unsigned)k * lsb);
         // element (al, beta = g) of the applied spin matrix of channel
         // a = 2 mt + am: forward W[qw][a][b], adjoint (W[qw][b][a])^+
 #pragma unroll
         for (int mt = 0; mt < NMT; ++mt) {
-          const int a = 2 * mt + am;
-          const int wa = ADJ ? b : a, wb = ADJ ? a : b;
-          e[mt] = W[(((size_t)qw * wcout + wa) * wcin + wb) * 16 + (ADJ ? g * 4 + al : al * 4 + g)];
+          const unsigned wblk = ADJ ? (unsigned)(((qw * wcout + b) * wcin + 2 * mt) * 16)
+                                    : (unsigned)(((qw * wcout + 2 * mt) * wcin + b) * 16);
+          e[mt] = ptc_ldu(rw, wlane, 4u * wblk);
         }
       };
       float wn[6];

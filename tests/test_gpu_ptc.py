@@ -30,8 +30,9 @@ def make(dims, Ls, seed, mq=0.05, M0=1.6, b=1.0, c=0.0):
     return U, op
 
 
-# channel counts 4 and 8 run the layer kernel's 4-channel groups, 6 its 2-channel
-# groups, 3 and 1 single channels (and the weight gradient's 2- and 1-channel groups)
+# even output channel counts run the matrix-core layer kernel (2, 3 and 4 tiles
+# for 4, 6 and 8 channels), odd ones the VALU kernel (and the weight gradient's
+# 2- and 1-channel groups)
 @pytest.mark.parametrize("dims,Ls,ch", [((4, 2, 2, 4), 4, (1, 4, 4, 1)),
                                         ((4, 2, 2, 4), 4, (1, 6, 3, 1)),
                                         ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1))])
@@ -52,7 +53,7 @@ def test_forward_matches_oracle(dims, Ls, ch):
     assert rel(op.reverse(fo), want) < 2e-5
 
 
-@pytest.mark.parametrize("ch", [(1, 4, 4, 1), (1, 6, 3, 1)])
+@pytest.mark.parametrize("ch", [(1, 4, 4, 1), (1, 6, 3, 1), (1, 8, 8, 1)])
 def test_loss_and_gradient_match_oracle(ch):
     print("case ptc gradient", ch, flush=True)
     dims, Ls = (4, 2, 2, 4), 4
