@@ -18,6 +18,9 @@
 //
 // Activation layout: channel-major, then parity, then the device spinor
 // layout of the operator, element ((site * 24 + k) * Ls + s), k = ri + 2 (c + 3 d).
+#include <cstdlib>
+#include <cstring>
+
 #include "dwf_device.h"
 #include "dwf_kernels.h"
 
@@ -162,8 +165,10 @@ __device__ __forceinline__ float ptc_ld(__amdgpu_buffer_rsrc_t r, unsigned voff,
   return __uint_as_float(ptc_ldu(r, voff, soff));
 }
 
-// Matrix-core form of the same layer for an even number of output channels:
-// per path q and input channel b the contraction is the real product
+// Matrix-core form of the same layer (any number of output channels up to 16;
+// an odd count leaves the second channel of the last tile with zero weights and
+// does not store it).  Per path q and input channel b the contraction is the
+// real product
 //   [out_re; out_im](a, alpha) += [Wr -Wi; Wi Wr](a, alpha; beta) [t_re; t_im](beta)
 // for every colour, computed as v_mfma_f32_16x16x4f32 tiles D(16x16) +=
 // A(16x4) B(4x16) with
@@ -179,7 +184,7 @@ __device__ __forceinline__ float ptc_ld(__amdgpu_buffer_rsrc_t r, unsigned voff,
 // three waves per SIMD (at most 168 registers) fit without spills up to NMT = 5
 template <bool ADJ, int NMT>
 __global__ void __launch_bounds__(256, NMT <= 5 ? 3 : 2)
-k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin,
+k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin, int cout,
                  const unsigned* __restrict__ W, int wcin, int wcout,
                  const float* __restrict__ links, const int* __restrict__ nbr, int Vh, int Ls) {
   // loads go through buffer resources with 32-bit byte offsets: per lane the
@@ -244,7 +249,8 @@ k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin,
         for (int mt = 0; mt < NMT; ++mt) {
           const unsigned wblk = ADJ ? (unsigned)(((qw * wcout + b) * wcin + 2 * mt) * 16)
                                     : (unsigned)(((qw * wcout + 2 * mt) * wcin + b) * 16);
-          e[mt] = ptc_ldu(rw, wlane, 4u * wblk);
+          // an odd cout pads the last tile with a zero channel
+          e[mt] = ptc_ldu(rw, 2 * mt + am < cout ? wlane : kOut, 4u * wblk);
         }
       };
       float wn[6];
@@ -302,6 +308,7 @@ k_ptc_layer_mfma(float* __restrict__ out, const float* __restrict__ in, int cin,
       // re / im r & 1
 #pragma unroll
       for (int mt = 0; mt < NMT; ++mt) {
+        if (2 * mt + (g >> 1) >= cout) continue;
         float* o = out + (size_t)(2 * mt + (g >> 1)) * 2 * fsz + (size_t)p * fsz +
                    (size_t)site * 24 * Ls + s;
 #pragma unroll
@@ -441,35 +448,42 @@ static void ptc_layer_launch(float* out, const float* in, int cin, int cout, con
 }
 
 template <bool ADJ, int NMT>
-static void ptc_layer_mfma_launch(float* out, const float* in, int cin, const unsigned* W,
-                                  int wcin, int wcout, const float* links, const int* nbr, int Vh,
-                                  int Ls, hipStream_t st) {
+static void ptc_layer_mfma_launch(float* out, const float* in, int cin, int cout,
+                                  const unsigned* W, int wcin, int wcout, const float* links,
+                                  const int* nbr, int Vh, int Ls, hipStream_t st) {
   const size_t tiles = 2 * (((size_t)Vh * Ls + 15) / 16);
   size_t blocks = (tiles + 3) / 4;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL((k_ptc_layer_mfma<ADJ, NMT>), dim3((unsigned)blocks), dim3(256), 0, st, out,
-                     in, cin, W, wcin, wcout, links, nbr, Vh, Ls);
+                     in, cin, cout, W, wcin, wcout, links, nbr, Vh, Ls);
 }
 
 template <bool ADJ>
 static void ptc_layer_dispatch(float* out, const float* in, int cin, int cout, const unsigned* W,
                                int wcin, int wcout, const float* links, const int* nbr, int Vh,
                                int Ls, hipStream_t st) {
-  // an even number of output channels: the matrix-core form, all channels of 16
-  // points per wave
-  switch (cout) {
-    case 2: return ptc_layer_mfma_launch<ADJ, 1>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 4: return ptc_layer_mfma_launch<ADJ, 2>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 6: return ptc_layer_mfma_launch<ADJ, 3>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 8: return ptc_layer_mfma_launch<ADJ, 4>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 10: return ptc_layer_mfma_launch<ADJ, 5>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 12: return ptc_layer_mfma_launch<ADJ, 6>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 14: return ptc_layer_mfma_launch<ADJ, 7>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    case 16: return ptc_layer_mfma_launch<ADJ, 8>(out, in, cin, W, wcin, wcout, links, nbr, Vh, Ls, st);
-    default: break;
+  // the matrix-core form (all output channels of 16 points per wave, an odd
+  // count padded with a zero channel) wherever its 32-bit buffer offsets fit:
+  // one channel's two parities, the links and the weights each below 2^31 bytes
+  // DWF_PTC_KERNEL=valu selects the VALU form everywhere (its tests use it)
+  const size_t fsz = (size_t)Vh * 24 * Ls;
+  const bool fits = 8 * fsz < (1ull << 31) && (size_t)2 * 8 * 18 * 4 * Vh < (1ull << 31);
+  const char* force = std::getenv("DWF_PTC_KERNEL");
+  if (fits && !(force && std::strcmp(force, "valu") == 0)) {
+    switch ((cout + 1) / 2) {
+#define PTC_MFMA(n)                                                                          \
+  case n:                                                                                    \
+    return ptc_layer_mfma_launch<ADJ, n>(out, in, cin, cout, W, wcin, wcout, links, nbr, Vh, \
+                                         Ls, st);
+      PTC_MFMA(1) PTC_MFMA(2) PTC_MFMA(3) PTC_MFMA(4)
+      PTC_MFMA(5) PTC_MFMA(6) PTC_MFMA(7) PTC_MFMA(8)
+#undef PTC_MFMA
+      default: break;
+    }
   }
   // otherwise the VALU form: four output channels per thread (227 registers, two
-  // waves per SIMD); eight need 324 registers, one wave per SIMD, and are slower
+  // waves per SIMD); eight need 324 registers, one wave per SIMD, and are slower;
+  // a thread per point and group, with 64-bit addressing
   if (cout % 4 == 0)
     ptc_layer_launch<ADJ, 4>(out, in, cin, cout, W, wcin, wcout, links, nbr, Vh, Ls, st);
   else if (cout % 2 == 0)

@@ -30,14 +30,18 @@ def make(dims, Ls, seed, mq=0.05, M0=1.6, b=1.0, c=0.0):
     return U, op
 
 
-# even output channel counts run the matrix-core layer kernel (2, 3 and 4 tiles
-# for 4, 6 and 8 channels), odd ones the VALU kernel (and the weight gradient's
-# 2- and 1-channel groups)
-@pytest.mark.parametrize("dims,Ls,ch", [((4, 2, 2, 4), 4, (1, 4, 4, 1)),
-                                        ((4, 2, 2, 4), 4, (1, 6, 3, 1)),
-                                        ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1))])
-def test_forward_matches_oracle(dims, Ls, ch):
-    print("case ptc forward", dims, Ls, ch, flush=True)
+# the matrix-core layer kernel with 2, 3 and 4 tiles (4, 6 and 8 channels; 3 and
+# 1 channels pad the last tile), and the VALU kernel that large lattices use
+# (4-, 2- and 1-channel groups); the weight gradient's 2- and 1-channel groups
+@pytest.mark.parametrize("dims,Ls,ch,kernel", [((4, 2, 2, 4), 4, (1, 4, 4, 1), "matrix"),
+                                               ((4, 2, 2, 4), 4, (1, 6, 3, 1), "matrix"),
+                                               ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1), "matrix"),
+                                               ((4, 2, 2, 4), 4, (1, 6, 3, 1), "valu"),
+                                               ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1), "valu")])
+def test_forward_matches_oracle(dims, Ls, ch, kernel, monkeypatch):
+    print("case ptc forward", dims, Ls, ch, kernel, flush=True)
+    if kernel == "valu":
+        monkeypatch.setenv("DWF_PTC_KERNEL", "valu")
     U, op = make(dims, Ls, 1)
     net = ParallelTransportNet(op, ch)
     rng = np.random.default_rng(2)
@@ -53,9 +57,13 @@ def test_forward_matches_oracle(dims, Ls, ch):
     assert rel(op.reverse(fo), want) < 2e-5
 
 
-@pytest.mark.parametrize("ch", [(1, 4, 4, 1), (1, 6, 3, 1), (1, 8, 8, 1)])
-def test_loss_and_gradient_match_oracle(ch):
-    print("case ptc gradient", ch, flush=True)
+@pytest.mark.parametrize("ch,kernel", [((1, 4, 4, 1), "matrix"), ((1, 6, 3, 1), "matrix"),
+                                       ((1, 8, 8, 1), "matrix"), ((1, 6, 3, 1), "valu"),
+                                       ((1, 8, 8, 1), "valu")])
+def test_loss_and_gradient_match_oracle(ch, kernel, monkeypatch):
+    print("case ptc gradient", ch, kernel, flush=True)
+    if kernel == "valu":
+        monkeypatch.setenv("DWF_PTC_KERNEL", "valu")
     dims, Ls = (4, 2, 2, 4), 4
     U, op = make(dims, Ls, 4)
     net = ParallelTransportNet(op, ch)
