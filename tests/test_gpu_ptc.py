@@ -36,6 +36,7 @@ def make(dims, Ls, seed, mq=0.05, M0=1.6, b=1.0, c=0.0):
 @pytest.mark.parametrize("dims,Ls,ch,kernel", [((4, 2, 2, 4), 4, (1, 4, 4, 1), "matrix"),
                                                ((4, 2, 2, 4), 4, (1, 6, 3, 1), "matrix"),
                                                ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1), "matrix"),
+                                               ((2, 2, 2, 6), 5, (1, 4, 2, 1), "matrix"),
                                                ((4, 2, 2, 4), 4, (1, 6, 3, 1), "valu"),
                                                ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1), "valu")])
 def test_forward_matches_oracle(dims, Ls, ch, kernel, monkeypatch):
@@ -57,14 +58,17 @@ def test_forward_matches_oracle(dims, Ls, ch, kernel, monkeypatch):
     assert rel(op.reverse(fo), want) < 2e-5
 
 
-@pytest.mark.parametrize("ch,kernel", [((1, 4, 4, 1), "matrix"), ((1, 6, 3, 1), "matrix"),
-                                       ((1, 8, 8, 1), "matrix"), ((1, 6, 3, 1), "valu"),
-                                       ((1, 8, 8, 1), "valu")])
-def test_loss_and_gradient_match_oracle(ch, kernel, monkeypatch):
-    print("case ptc gradient", ch, kernel, flush=True)
+# 2x2x2x6 at Ls = 5: 120 points per parity, so the last 16-point tile is partial
+@pytest.mark.parametrize("dims,Ls,ch,kernel", [((4, 2, 2, 4), 4, (1, 4, 4, 1), "matrix"),
+                                               ((4, 2, 2, 4), 4, (1, 6, 3, 1), "matrix"),
+                                               ((4, 2, 2, 4), 4, (1, 8, 8, 1), "matrix"),
+                                               ((2, 2, 2, 6), 5, (1, 4, 2, 1), "matrix"),
+                                               ((4, 2, 2, 4), 4, (1, 6, 3, 1), "valu"),
+                                               ((4, 2, 2, 4), 4, (1, 8, 8, 1), "valu")])
+def test_loss_and_gradient_match_oracle(dims, Ls, ch, kernel, monkeypatch):
+    print("case ptc gradient", dims, Ls, ch, kernel, flush=True)
     if kernel == "valu":
         monkeypatch.setenv("DWF_PTC_KERNEL", "valu")
-    dims, Ls = (4, 2, 2, 4), 4
     U, op = make(dims, Ls, 4)
     net = ParallelTransportNet(op, ch)
     rng = np.random.default_rng(5)
