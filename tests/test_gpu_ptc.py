@@ -31,12 +31,15 @@ def make(dims, Ls, seed, mq=0.05, M0=1.6, b=1.0, c=0.0):
 
 
 # the matrix-core layer kernel with 2, 3 and 4 tiles (4, 6 and 8 channels; 3 and
-# 1 channels pad the last tile), and the VALU kernel that large lattices use
+# 1 channels pad the last tile; at Ls = 16 a 16-point tile is one site, at Ls = 12
+# tiles straddle sites), and the VALU kernel that large lattices use
 # (4-, 2- and 1-channel groups); the weight gradient's 2- and 1-channel groups
 @pytest.mark.parametrize("dims,Ls,ch,kernel", [((4, 2, 2, 4), 4, (1, 4, 4, 1), "matrix"),
                                                ((4, 2, 2, 4), 4, (1, 6, 3, 1), "matrix"),
                                                ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1), "matrix"),
                                                ((2, 2, 2, 6), 5, (1, 4, 2, 1), "matrix"),
+                                               ((4, 2, 2, 4), 12, (1, 4, 4, 1), "matrix"),
+                                               ((4, 2, 2, 4), 16, (1, 8, 8, 1), "matrix"),
                                                ((4, 2, 2, 4), 4, (1, 6, 3, 1), "valu"),
                                                ((4, 4, 2, 4), 8, (1, 8, 8, 8, 1), "valu")])
 def test_forward_matches_oracle(dims, Ls, ch, kernel, monkeypatch):
